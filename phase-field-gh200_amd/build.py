@@ -1,0 +1,71 @@
+"""Build recipe of the gfx950 library (lib/libpfh.so) and the pf2d CLI.
+
+Compiles every HIP source under csrc/ for gfx950 and links it with the C++
+host driver into one shared library exporting the C ABI of include/pfh.h.
+Runs on a machine without a GPU (hipcc cross-compiles).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent
+CSRC = HERE / "csrc"
+LIB = HERE / "lib"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: binary32 arithmetic in source order, as the reference's
+# host semantics define it. Packed FP32 instructions are disabled on purpose.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", str(REPO / "include"), "-I", str(CSRC),
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:4])} ...")
+
+
+def _stale(out: Path, deps: list[Path]) -> bool:
+    if not out.exists():
+        return True
+    t = out.stat().st_mtime
+    return any(d.stat().st_mtime > t for d in deps)
+
+
+def build(force: bool = True) -> Path:
+    """Compile csrc/ into lib/libpfh.so (and lib/pf2d). Returns the .so path."""
+    LIB.mkdir(exist_ok=True)
+    hip_srcs = sorted(CSRC.glob("*.hip"))
+    cpp_srcs = sorted(CSRC.glob("*.cpp"))
+    headers = sorted(CSRC.glob("*.h")) + sorted(CSRC.glob("*.hpp")) + [REPO / "include" / "pfh.h"]
+    objs = []
+    for src in hip_srcs + cpp_srcs:
+        obj = LIB / (src.stem + (".hip.o" if src.suffix == ".hip" else ".o"))
+        if force or _stale(obj, [src] + headers):
+            lang = [] if src.suffix == ".hip" else ["-x", "hip"]
+            _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(src), "-o", str(obj)])
+        objs.append(obj)
+    so = LIB / "libpfh.so"
+    tmp = LIB / "libpfh.so.tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
+          *map(str, objs)])
+    os.replace(tmp, so)
+    main_src = HERE / "tools" / "pf2d_main.cpp"
+    if main_src.exists():
+        exe = LIB / "pf2d"
+        _run([HIPCC, *HIP_FLAGS, "-x", "hip", str(main_src), "-o", str(exe),
+              "-L", str(LIB), "-lpfh", f"-Wl,-rpath,{LIB}"])
+    return so
+
+
+if __name__ == "__main__":
+    print(build(force="--incremental" not in sys.argv))

@@ -1,0 +1,1258 @@
+// gfx950 kernels of the 2D per-cell phase-field update and their C ABI.
+//
+// The step is one persistent work-queue launch: one 1024-thread workgroup per
+// compute unit takes cell tickets from a device cursor (reference step_body,
+// src/kernels.cu:948-1024) and advances one cell at a time through:
+//   A  stage the cell's active window of phi in LDS (16 B vector loads) and
+//      hold its S window in VGPRs; a column-walk 3x3 register stencil gives
+//      the interaction integral terms and, for every pixel, the passive
+//      right-hand side and both gradients, which stay in VGPRs;
+//   -  fixed-order fp64 reduction (64-lane tree, 16 wavefronts ascending) of
+//      the interaction integral and the cell velocity;
+//   B  the explicit Euler update from the held terms plus advection, the
+//      shifted store, the Q5.27 scatter into the next S with integer atomics,
+//      and the moments/support/phi_max of the stored field;
+//   -  a second fixed-order reduction and the CellState commit.
+// Classes 0-3 take the LDS path; classes 4 (224x224, larger than gfx950's
+// 160 KB LDS once fp32 phi is staged) and 5 (fallback) read the tile from
+// global memory.
+//
+// Arithmetic follows the reference's host semantics exactly: binary32
+// operations in source order with no multiply-add contraction (the build
+// passes -ffp-contract=off), fp64 reductions, Q5.27 integer scatter.
+// Reference: process_cell src/kernels.cu:53-592, process_cell_fallback
+// :604-937, init/verify/pack/morton :1044-1302.
+
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/pfh.h"
+#include "pfh_device.h"
+
+namespace pfh {
+
+// Control block written by thread 0 and read by the workgroup.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct Ctl {
+    int sx, sy, gx0, gy0, gx0n, gy0n, dcls, zero_tile, fm, pctr, tumbled;
+    int dwx, dwy, dtx0, dty0, no_margin;
+    float gam, dwC, volC, vA, cosp, sinp, theta, mrel, vx, vy;
+};
+
+struct Smem {
+    pfh2d_step_args args;        // LDS copy for the out-of-line class paths
+    double red[kWaves * 8];
+    int ired[kWaves * 8];
+    Ctl ctl;
+    int q_n, q_cls;
+    float phi[kPhiLdsFloats];
+};
+
+// threadIdx.x made opaque inside the persistent loop: without this, LICM
+// hoists every per-thread index and address of the cell body out of the loop
+// and keeps them all live (and spilled) across cells.
+__device__ __forceinline__ int opaque_tid() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+__device__ __forceinline__ void flag_add(uint32_t* flags, int i, uint32_t v) {
+    atomicAdd(&flags[i], v);
+}
+__device__ __forceinline__ void flag_or(uint32_t* flags, int i) {
+    atomicOr(&flags[i], 1u);
+}
+
+// Global (address space 1) views of the hot buffers: pointers read out of the
+// argument struct are otherwise generic, and flat instructions would tie every
+// LDS wait to outstanding memory operations.
+#define PFH_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ PFH_GLOBAL T* gptr(T* p) {
+    return (PFH_GLOBAL T*)p;
+}
+__device__ __forceinline__ uint32_t atomic_add_g(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The step arguments cell_scalars reads, passed by value so that the kernel
+// argument block never has its address taken (which copies it to scratch).
+struct ScalarArgs {
+    const pfh_cell_state* cell;
+    uint32_t* flags;
+    unsigned long long polarity_seed;
+    double p_tumble, vol_scale, A0;
+    float bulk_scale;
+    int L, parity_out, full_moment_every;
+};
+
+__device__ __forceinline__ ScalarArgs scalar_args(const pfh2d_step_args& A) {
+    ScalarArgs s;
+    s.cell = A.cell;
+    s.flags = A.flags;
+    s.polarity_seed = A.polarity_seed;
+    s.p_tumble = A.p_tumble;
+    s.vol_scale = A.vol_scale;
+    s.A0 = A.A0;
+    s.bulk_scale = A.bulk_scale;
+    s.L = A.L;
+    s.parity_out = A.parity_out;
+    s.full_moment_every = A.full_moment_every;
+    return s;
+}
+
+// Per-cell scalars: tumble, polarity, coefficients, class hysteresis and the
+// recentring shift (src/kernels.cu:93-191 and :624-711).
+// Out of line: its double-precision polynomial constants must not be hoisted
+// out of the persistent loop into the update's register budget.
+template <int CLS>
+__device__ __noinline__ void cell_scalars(const ScalarArgs A, int n,
+                                         unsigned long long step, Ctl* ctl) {
+    constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
+    const pfh_cell_state& cs = A.cell[n];
+    const Philox4 r = philox4x32_10(
+        (uint32_t)(step & 0xFFFFFFFFull), (uint32_t)(step >> 32),
+        (uint32_t)cs.global_id, 0xA5A5A5A5u,
+        (uint32_t)(A.polarity_seed & 0xFFFFFFFFull),
+        (uint32_t)(A.polarity_seed >> 32));
+    float theta = cs.theta;
+    int tumbled = 0;
+    if (uniform53(r.v[0], r.v[1]) < A.p_tumble) {
+        theta = (float)(2.0 * kPi * uniform53(r.v[2], r.v[3]));
+        tumbled = 1;
+    }
+    double sd, cd;
+    sincos_d((double)theta, &sd, &cd);
+    const float gam = cs.gamma;
+    const float dwC = A.bulk_scale * gam;
+    const float volC = (float)(A.vol_scale * (A.A0 - cs.V));
+    const float mrel = cs.M_pf * (float)(1.0 / 0.5);
+
+    const int ex = cs.bb_hi_x - cs.bb_lo_x + 1;
+    const int ey = cs.bb_hi_y - cs.bb_lo_y + 1;
+    int dcls = CLS, no_margin = 0;
+    unsigned pctr = cs.promote_ctr;
+    if (ex > 0) {
+        if (ex + kPromoteSlack > WX || ey + kPromoteSlack > WY) {
+            const int grow = class_containing(ex, ey, kPromoteSlack);
+            if (grow >= 0) {
+                dcls = grow;
+            } else if (CLS == kClassFallback && ex <= WX && ey <= WY) {
+                no_margin = 1;
+            } else {
+                flag_add(A.flags, PFH_FLAG_CLASS_EXHAUSTED, 1u);
+            }
+            pctr = 0u;
+        } else {
+            const int small = class_containing(ex, ey, kDemoteSlack);
+            if (small >= 0 && small != CLS &&
+                shape_of(small).wx * shape_of(small).wy < WX * WY) {
+                if (++pctr >= (unsigned)kDemoteDwell) { dcls = small; pctr = 0u; }
+            } else {
+                pctr = 0u;
+            }
+        }
+    }
+    const Shape dc = shape_of(dcls);
+    int sx = 0, sy = 0;
+    if (cs.V > 0.0) {
+        const double cxr = cs.Cx / cs.V;
+        const double cyr = cs.Cy / cs.V;
+        sx = __double2int_rn(cxr - 0.5 * (double)(dc.wx - 1));
+        sy = __double2int_rn(cyr - 0.5 * (double)(dc.wy - 1));
+        if (dcls == CLS) {
+            sx = max(-kMaxShiftPerStep, min(kMaxShiftPerStep, sx));
+            sy = max(-kMaxShiftPerStep, min(kMaxShiftPerStep, sy));
+        } else {
+            sx = max(-WX, min(WX, sx));
+            sy = max(-WY, min(WY, sy));
+        }
+    } else {
+        flag_or(A.flags, PFH_FLAG_V_NONPOS);
+    }
+    ctl->sx = sx;
+    ctl->sy = sy;
+    ctl->gx0 = cs.gx0;
+    ctl->gy0 = cs.gy0;
+    ctl->gx0n = wrapi(cs.gx0 + sx, A.L);
+    ctl->gy0n = wrapi(cs.gy0 + sy, A.L);
+    ctl->dcls = dcls;
+    ctl->zero_tile = (dcls != (int)cs.cls_written[A.parity_out]);
+    ctl->fm = (A.full_moment_every > 0 &&
+               ((step + 1ull) % (unsigned long long)A.full_moment_every) == 0ull);
+    ctl->pctr = (int)pctr;
+    ctl->tumbled = tumbled;
+    ctl->dwx = dc.wx;
+    ctl->dwy = dc.wy;
+    ctl->dtx0 = dc.tx0;
+    ctl->dty0 = dc.ty0;
+    ctl->no_margin = no_margin;
+    ctl->gam = gam;
+    ctl->dwC = dwC;
+    ctl->volC = volC;
+    ctl->vA = cs.v_A;
+    ctl->cosp = (float)cd;
+    ctl->sinp = (float)sd;
+    ctl->theta = theta;
+    ctl->mrel = mrel;
+}
+
+// Zero every tile pixel outside the destination window (class change). The
+// window itself is fully rewritten by the update, so no ordering is needed.
+__device__ void zero_outside_window(float* tile, int dwx, int dwy, int dtx0,
+                                    int dty0) {
+    for (int i = opaque_tid(); i < kTileArea; i += kNT) {
+        const int x = i % kTilePitch, y = i / kTilePitch;
+        if (!((unsigned)(x - dtx0) < (unsigned)dwx &&
+              (unsigned)(y - dty0) < (unsigned)dwy))
+            tile[i] = 0.0f;
+    }
+}
+
+// Velocity from the interaction integral (src/kernels.cu:300-342): wave
+// trees, then thread 0 adds the 16 wave sums in order and commits.
+__device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
+                                double aIx, double aIy) {
+    const int tid = opaque_tid();
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    aIx = wave_sum(aIx);
+    aIy = wave_sum(aIy);
+    if (lane == 0) {
+        sm.red[wave * 8 + 0] = aIx;
+        sm.red[wave * 8 + 1] = aIy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sIx = 0.0, sIy = 0.0;
+        for (int w = 0; w < kWaves; ++w) {
+            sIx += sm.red[w * 8 + 0];
+            sIy += sm.red[w * 8 + 1];
+        }
+        const double vxd = (double)sm.ctl.vA * (double)sm.ctl.cosp
+                         + (double)A.mot_coeff * sIx;
+        const double vyd = (double)sm.ctl.vA * (double)sm.ctl.sinp
+                         + (double)A.mot_coeff * sIy;
+        sm.ctl.vx = (float)vxd;
+        sm.ctl.vy = (float)vyd;
+        pfh_cell_state* cs = &A.cell[n];
+        cs->vx = (float)vxd;
+        cs->vy = (float)vyd;
+        cs->Ix = sIx;
+        cs->Iy = sIy;
+    }
+    __syncthreads();
+}
+
+// Moments/support reduction and CellState commit (src/kernels.cu:542-591).
+template <int CLS>
+__device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
+                              double aV, double aCx, double aCy, double aPer,
+                              int blox, int bhix, int bloy, int bhiy,
+                              int pmaxb) {
+    const int tid = opaque_tid();
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    aV = wave_sum(aV);
+    aCx = wave_sum(aCx);
+    aCy = wave_sum(aCy);
+    aPer = wave_sum(aPer);
+    blox = wave_min(blox);
+    bhix = wave_max(bhix);
+    bloy = wave_min(bloy);
+    bhiy = wave_max(bhiy);
+    pmaxb = wave_max(pmaxb);
+    if (lane == 0) {
+        double* rw = sm.red + wave * 8;
+        rw[0] = aV; rw[1] = aCx; rw[2] = aCy; rw[3] = aPer;
+        int* iw = sm.ired + wave * 8;
+        iw[0] = blox; iw[1] = bhix; iw[2] = bloy; iw[3] = bhiy; iw[4] = pmaxb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const Ctl& c = sm.ctl;
+        double sV = 0.0, sCx = 0.0, sCy = 0.0, sPer = 0.0;
+        int Blox = c.dwx, Bhix = -1, Bloy = c.dwy, Bhiy = -1, Pmax = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const double* rw = sm.red + w * 8;
+            sV += rw[0]; sCx += rw[1]; sCy += rw[2]; sPer += rw[3];
+            const int* iw = sm.ired + w * 8;
+            Blox = min(Blox, iw[0]); Bhix = max(Bhix, iw[1]);
+            Bloy = min(Bloy, iw[2]); Bhiy = max(Bhiy, iw[3]);
+            Pmax = max(Pmax, iw[4]);
+        }
+        pfh_cell_state* cs = &A.cell[n];
+        cs->gx0 = c.gx0n;
+        cs->gy0 = c.gy0n;
+        cs->cls = (uint8_t)c.dcls;
+        cs->cls_written[A.parity_out] = (uint8_t)c.dcls;
+        cs->theta = c.theta;
+        cs->V = sV;
+        cs->Cx = sCx;
+        cs->Cy = sCy;
+        if (c.fm) cs->perim = sPer;
+        cs->bb_lo_x = Blox;
+        cs->bb_hi_x = Bhix;
+        cs->bb_lo_y = Bloy;
+        cs->bb_hi_y = Bhiy;
+        cs->promote_ctr = (uint32_t)c.pctr;
+        cs->shift_ctr += (c.sx | c.sy) ? 1u : 0u;
+        cs->tumble_ctr += (uint32_t)c.tumbled;
+        cs->phi_max = __uint_as_float((uint32_t)Pmax);
+        A.cell_cls[n] = (uint8_t)c.dcls;
+        if (CLS == kClassFallback) {
+            cs->reserved[0] = 1u;
+            const bool edge = Bhix >= 0 && (Blox == 0 || Bhix == c.dwx - 1 ||
+                                            Bloy == 0 || Bhiy == c.dwy - 1);
+            if (c.no_margin || (c.dcls == kClassFallback && edge))
+                ++cs->reserved[1];
+        }
+    }
+    __syncthreads();
+}
+
+// -------------------------------------------------------------------------
+// LDS path: classes 0-3.
+// -------------------------------------------------------------------------
+
+// Column walk over LDS rows: thread (x, band) visits rows y0..y0+RB-1 of
+// column x with a rolling 3x3 neighbourhood (three LDS reads per row).
+struct Roll {
+    float sW, sC, sE, cW, cC, cE, nW, nC, nE;
+};
+
+template <int WX>
+__device__ __forceinline__ void roll_row(const float* col, int r, bool hasW,
+                                         bool hasE, float& w, float& c, float& e) {
+    const float* p = col + r * WX;
+    c = p[0];
+    const float wr = p[-1], er = p[1];
+    w = hasW ? wr : 0.0f;
+    e = hasE ? er : 0.0f;
+}
+
+// Passive right-hand side of phase_field_rhs (include/kernels.cuh:127-136),
+// source order, no contraction. Also returns S_other via So.
+__device__ __forceinline__ float passive_rhs(const Roll& q, uint32_t qS,
+                                             float gam, float dwC, float volC,
+                                             float repC, float& So,
+                                             uint32_t& negc) {
+    const float lap = (4.0f * (q.nC + q.sC + q.cE + q.cW)
+                       + 1.0f * (q.nE + q.nW + q.sE + q.sW)
+                       + -20.0f * q.cC) * (float)(1.0 / 6.0);
+    const uint32_t qc = q_of(q.cC);
+    So = 0.0f;
+    if (qS >= qc) So = (float)(qS - qc) * kQInv;
+    else negc += 2u;     // the reference evaluates S_other once per pass
+    return gam * lap
+         - dwC * (q.cC * (1.0f - q.cC) * (1.0f - 2.0f * q.cC))
+         + volC * q.cC
+         - repC * q.cC * So;
+}
+
+template <int CLS>
+__device__ void cell_staged(const pfh2d_step_args& A, int n,
+                            unsigned long long step, Smem& sm) {
+    using G = Staged<CLS>;
+    constexpr int WX = G::WX, WY = G::WY, TX0 = G::TX0, TY0 = G::TY0;
+    constexpr int RB = G::RB, NCH = G::NCH, NCHUNK = G::NCHUNK, KCH = G::KCH;
+    const int tid = opaque_tid();
+    const int x = tid % WX;
+    const int band = tid / WX;
+    const bool active = band < G::NB;
+    const int y0 = band * RB;
+    const int L = A.L, P = A.P;
+    const int gx0 = A.cell[n].gx0;
+    const int gy0 = A.cell[n].gy0;
+    float* phi_s = sm.phi + 4;   // 4-float guard keeps x-1 / x+1 in bounds
+    const float* col = phi_s + x;
+    const bool hasW = x > 0, hasE = x < WX - 1;
+    const int gx = (gx0 + x < L) ? gx0 + x : gx0 + x - L;
+
+    // Thread 0 first, while nothing else is live across its call; the other
+    // fifteen wavefronts go straight on to issue their loads.
+    if (tid == 0) cell_scalars<CLS>(scalar_args(A), n, step, &sm.ctl);
+
+    // Window rows of phi in flight (16 B each) ...
+    const PFH_GLOBAL float* tin = gptr(A.phi_in) + (size_t)n * kTileArea;
+    const PFH_GLOBAL uint32_t* S_rd = gptr(A.S_rd);
+    f32x4 ph[KCH];
+#pragma unroll
+    for (int k = 0; k < KCH; ++k) {
+        const int u = tid + k * kNT;
+        if (u < NCHUNK) {
+            const int r = u / NCH, c4 = u - (u / NCH) * NCH;
+            ph[k] = *reinterpret_cast<const PFH_GLOBAL f32x4*>(
+                tin + (size_t)(TY0 + r) * kTilePitch + TX0 + 4 * c4);
+        }
+    }
+    // ... and this thread's column of the S window.
+    auto load_qs = [&](uint32_t (&qS)[RB]) {
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            const int y = y0 + k;
+            qS[k] = 0u;
+            if (active && y < WY) {
+                int gy = gy0 + y;
+                if (gy >= L) gy -= L;
+                qS[k] = S_rd[(size_t)gy * P + gx];
+            }
+        }
+    };
+    uint32_t qS[RB];
+    load_qs(qS);
+#pragma unroll
+    for (int k = 0; k < KCH; ++k) {
+        const int u = tid + k * kNT;
+        if (u < NCHUNK) {
+            const int r = u / NCH, c4 = u - (u / NCH) * NCH;
+            *reinterpret_cast<f32x4*>(phi_s + (r + 1) * WX + 4 * c4) = ph[k];
+        }
+    }
+    for (int i = tid; i < WX; i += kNT) {
+        phi_s[i] = 0.0f;                     // row -1
+        phi_s[(WY + 1) * WX + i] = 0.0f;     // row WY
+    }
+    __syncthreads();
+
+    const Ctl& c = sm.ctl;
+    const float gam = c.gam, dwC = c.dwC, volC = c.volC;
+    const float repC = A.rep_coeff;
+    if (c.zero_tile)
+        zero_outside_window(A.phi_out + (size_t)n * kTileArea, c.dwx, c.dwy,
+                            c.dtx0, c.dty0);
+
+    // ---- phase A: interaction integral and the held passive RHS ----
+    float pas[RB];
+    double aIx = 0.0, aIy = 0.0;
+    uint32_t negc = 0u;
+    if (active) {
+        Roll q;
+        roll_row<WX>(col, y0, hasW, hasE, q.sW, q.sC, q.sE);       // row y0-1
+        roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);   // row y0
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            pas[k] = 0.0f;
+            if (y0 + k < WY) {
+                roll_row<WX>(col, y0 + k + 2, hasW, hasE, q.nW, q.nC, q.nE);
+                float So;
+                pas[k] = passive_rhs(q, qS[k], gam, dwC, volC, repC, So, negc);
+                const float gxv = 0.5f * (q.cE - q.cW);
+                const float gyv = 0.5f * (q.nC - q.sC);
+                aIx += (double)(q.cC * gxv * So);
+                aIy += (double)(q.cC * gyv * So);
+                q.sW = q.cW; q.sC = q.cC; q.sE = q.cE;
+                q.cW = q.nW; q.cC = q.nC; q.cE = q.nE;
+            }
+            // One row per scheduling region keeps the walk's register
+            // footprint at one row instead of the whole column.
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (negc) flag_add(A.flags, PFH_FLAG_S_NEGATIVE, negc);
+
+    reduce_velocity(A, n, sm, aIx, aIy);
+
+    // ---- phase B: Euler update, shifted store, scatter, moments ----
+    const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
+    const int sx = c.sx, sy = c.sy, dwx = c.dwx, dwy = c.dwy;
+    const int dtx0 = c.dtx0, dty0 = c.dty0;
+    PFH_GLOBAL float* tout = gptr(A.phi_out) + (size_t)n * kTileArea;
+    double aV = 0.0, aCx = 0.0, aCy = 0.0, aPer = 0.0;
+    int blox = dwx, bhix = -1, bloy = dwy, bhiy = -1, pmaxb = 0;
+    bool nonfin = false, clampf = false, ovf = false;
+    auto update = [&](const Roll& q, float pas_k) -> float {
+        const float gxv = 0.5f * (q.cE - q.cW);
+        const float gyv = 0.5f * (q.nC - q.sC);
+        const float adv = vx * gxv + vy * gyv;
+        const float rhs = (mrel == 1.0f) ? pas_k - adv : mrel * pas_k - adv;
+        return q.cC + dt * rhs;
+    };
+    if (active) {
+        const int a = x - sx;
+        const bool ain = (unsigned)a < (unsigned)dwx;
+        uint32_t olds[RB];
+        Roll q;
+        q.sW = q.sE = q.nW = q.nE = 0.0f;
+        q.sC = col[y0 * WX];
+        roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            const int y = y0 + k;
+            olds[k] = 0u;
+            if (y < WY) {
+                float nW, nE;
+                roll_row<WX>(col, y + 2, hasW, hasE, nW, q.nC, nE);
+                const float pn = update(q, pas[k]);
+                const int b = y - sy;
+                if (ain && (unsigned)b < (unsigned)dwy) {
+                    tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = pn;
+                    nonfin |= !isfinite(pn);
+                    clampf |= (pn * pn > kQClampPhiSq);
+                    pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
+                    const uint32_t qv = q_of(pn);
+                    if (qv) {
+                        int gy = gy0 + y;
+                        if (gy >= L) gy -= L;
+                        olds[k] = atomic_add_g(&A.S_sc[(size_t)gy * P + gx], qv);
+                    }
+                    const double d = (double)pn * (double)pn;
+                    aV += d;
+                    aCx += d * (double)a;
+                    aCy += d * (double)b;
+                    if (pn > kSupportEps) {
+                        blox = min(blox, a); bhix = max(bhix, a);
+                        bloy = min(bloy, b); bhiy = max(bhiy, b);
+                    }
+                }
+                q.sC = q.cC;
+                q.cW = nW; q.cC = q.nC; q.cE = nE;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Exact S_overflow test with one register per pixel: every addend is
+        // at most 2^29, so a wrap needs old >= 2^32 - 2^29; only such
+        // candidates recompute q from this thread's own stored value.
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            if (olds[k] >= 0xE0000000u) {
+                const int b = y0 + k - sy;
+                const uint32_t qv =
+                    q_of(tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a]);
+                ovf |= olds[k] > 0xFFFFFFFFu - qv;
+            }
+        }
+    }
+    if (nonfin) flag_or(A.flags, PFH_FLAG_NONFINITE);
+    if (clampf) flag_or(A.flags, PFH_FLAG_Q_CLAMP);
+    if (ovf) flag_or(A.flags, PFH_FLAG_S_OVERFLOW);
+
+    // Destination pixels whose source lies outside the window are zero.
+    if (c.dcls == CLS) {
+        const int ny = sy > 0 ? sy : -sy, nx = sx > 0 ? sx : -sx;
+        const int rows = ny * dwx, total = rows + nx * dwy;
+        for (int i = tid; i < total; i += kNT) {
+            int a, b;
+            if (i < rows) {
+                const int rr = i / dwx;
+                a = i - rr * dwx;
+                b = sy > 0 ? dwy - sy + rr : rr;
+            } else {
+                const int j = i - rows, cc = j / dwy;
+                b = j - cc * dwy;
+                a = sx > 0 ? dwx - sx + cc : cc;
+            }
+            tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = 0.0f;
+        }
+    } else {
+        for (int i = tid; i < dwx * dwy; i += kNT) {
+            const int b = i / dwx, a = i - (i / dwx) * dwx;
+            const int ii = a + sx, jj = b + sy;
+            if (!((unsigned)ii < (unsigned)WX && (unsigned)jj < (unsigned)WY))
+                tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = 0.0f;
+        }
+    }
+
+    // Perimeter on full-moment steps (one step in full_moment_every) from the
+    // new field in the source layout with source-window zeros (the staged
+    // semantics, src/kernels.cu:512-538). phi_next is recomputed here, bit for
+    // bit as above, instead of being held through the update.
+    if (c.fm) {
+        float pns[RB];
+        if (active) {
+            uint32_t qS2[RB];
+            load_qs(qS2);
+            uint32_t negc2 = 0u;
+            Roll q;
+            roll_row<WX>(col, y0, hasW, hasE, q.sW, q.sC, q.sE);
+            roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);
+#pragma unroll
+            for (int k = 0; k < RB; ++k) {
+                pns[k] = 0.0f;
+                if (y0 + k < WY) {
+                    roll_row<WX>(col, y0 + k + 2, hasW, hasE, q.nW, q.nC, q.nE);
+                    float So;
+                    const float pk = passive_rhs(q, qS2[k], gam, dwC, volC, repC, So, negc2);
+                    pns[k] = update(q, pk);
+                    q.sW = q.cW; q.sC = q.cC; q.sE = q.cE;
+                    q.cW = q.nW; q.cC = q.nC; q.cE = q.nE;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();                         // all reads of phi_s done
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < RB; ++k)
+                if (y0 + k < WY) phi_s[(y0 + k + 1) * WX + x] = pns[k];
+        }
+        __syncthreads();
+        for (int i = tid; i < dwx * dwy; i += kNT) {
+            const int b = i / dwx, a = i - (i / dwx) * dwx;
+            const int ii = a + sx, jj = b + sy;
+            const bool iin = (unsigned)ii < (unsigned)WX;
+            const bool jin = (unsigned)jj < (unsigned)WY;
+            float e = 0.0f, w = 0.0f, nn = 0.0f, ss = 0.0f;
+            if (jin) {
+                if ((unsigned)(ii + 1) < (unsigned)WX) e = phi_s[(jj + 1) * WX + ii + 1];
+                if ((unsigned)(ii - 1) < (unsigned)WX) w = phi_s[(jj + 1) * WX + ii - 1];
+            }
+            if (iin) {
+                if ((unsigned)(jj + 1) < (unsigned)WY) nn = phi_s[(jj + 2) * WX + ii];
+                if ((unsigned)(jj - 1) < (unsigned)WY) ss = phi_s[jj * WX + ii];
+            }
+            const float pgx = 0.5f * (e - w), pgy = 0.5f * (nn - ss);
+            aPer += (double)sqrtf(pgx * pgx + pgy * pgy);
+        }
+    }
+    reduce_commit<CLS>(A, n, sm, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
+                       pmaxb);
+}
+
+// -------------------------------------------------------------------------
+// Global-memory path: class 4 (window zeros synthesized) and the fallback
+// class 5 (the tile's own one-pixel zero ring), src/kernels.cu:604-937.
+// -------------------------------------------------------------------------
+template <int CLS>
+__device__ void cell_global(const pfh2d_step_args& A, int n,
+                            unsigned long long step, Smem& sm) {
+    constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
+    constexpr int TX0 = kShapes[CLS].tx0, TY0 = kShapes[CLS].ty0;
+    const int tid = opaque_tid();
+    const int L = A.L, P = A.P;
+    if (tid == 0) cell_scalars<CLS>(scalar_args(A), n, step, &sm.ctl);
+    __syncthreads();
+    const Ctl& c = sm.ctl;
+    const int gx0 = c.gx0, gy0 = c.gy0;
+    const float* tin = A.phi_in + (size_t)n * kTileArea;
+    float* tout = A.phi_out + (size_t)n * kTileArea;
+    auto pin = [&](int xx, int yy) -> float {
+        if (CLS != kClassFallback &&
+            !((unsigned)xx < (unsigned)WX && (unsigned)yy < (unsigned)WY))
+            return 0.0f;
+        return tin[(size_t)(TY0 + yy) * kTilePitch + TX0 + xx];
+    };
+    if (c.zero_tile) zero_outside_window(tout, c.dwx, c.dwy, c.dtx0, c.dty0);
+
+    double aIx = 0.0, aIy = 0.0;
+    uint32_t negc = 0u;
+    for (int p = tid; p < WX * WY; p += kNT) {
+        const int y = p / WX, x = p - (p / WX) * WX;
+        const float cc = pin(x, y);
+        const float gxv = 0.5f * (pin(x + 1, y) - pin(x - 1, y));
+        const float gyv = 0.5f * (pin(x, y + 1) - pin(x, y - 1));
+        const uint32_t qS = A.S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
+        const uint32_t qc = q_of(cc);
+        float So = 0.0f;
+        if (qS >= qc) So = (float)(qS - qc) * kQInv;
+        else negc += 1u;
+        aIx += (double)(cc * gxv * So);
+        aIy += (double)(cc * gyv * So);
+    }
+    reduce_velocity(A, n, sm, aIx, aIy);
+
+    const float gam = c.gam, dwC = c.dwC, volC = c.volC, repC = A.rep_coeff;
+    const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
+    const int sx = c.sx, sy = c.sy, dwx = c.dwx, dwy = c.dwy;
+    const int dtx0 = c.dtx0, dty0 = c.dty0;
+    for (int p = tid; p < WX * WY; p += kNT) {
+        const int y = p / WX, x = p - (p / WX) * WX;
+        const float cC = pin(x, y);
+        const float cE = pin(x + 1, y), cW = pin(x - 1, y);
+        const float nC = pin(x, y + 1), sC = pin(x, y - 1);
+        const float nE = pin(x + 1, y + 1), nW = pin(x - 1, y + 1);
+        const float sE = pin(x + 1, y - 1), sW = pin(x - 1, y - 1);
+        const float lap = (4.0f * (nC + sC + cE + cW)
+                           + 1.0f * (nE + nW + sE + sW)
+                           + -20.0f * cC) * (float)(1.0 / 6.0);
+        const float gxv = 0.5f * (cE - cW);
+        const float gyv = 0.5f * (nC - sC);
+        const uint32_t qS = A.S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
+        const uint32_t qc = q_of(cC);
+        float So = 0.0f;
+        if (qS >= qc) So = (float)(qS - qc) * kQInv;
+        else negc += 1u;
+        const float passive = gam * lap
+            - dwC * (cC * (1.0f - cC) * (1.0f - 2.0f * cC))
+            + volC * cC
+            - repC * cC * So;
+        const float adv = vx * gxv + vy * gyv;
+        const float rhs = (mrel == 1.0f) ? passive - adv : mrel * passive - adv;
+        const float pn = cC + dt * rhs;
+        const int a = x - sx, b = y - sy;
+        if ((unsigned)a < (unsigned)dwx && (unsigned)b < (unsigned)dwy)
+            tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = pn;
+    }
+    if (negc) flag_add(A.flags, PFH_FLAG_S_NEGATIVE, negc);
+    for (int i = tid; i < dwx * dwy; i += kNT) {
+        const int b = i / dwx, a = i - (i / dwx) * dwx;
+        const int ii = a + sx, jj = b + sy;
+        if (!((unsigned)ii < (unsigned)WX && (unsigned)jj < (unsigned)WY))
+            tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = 0.0f;
+    }
+    __threadfence();      // stores reach L2; L1 lines are invalidated
+    __syncthreads();
+
+    const int gx0n = c.gx0n, gy0n = c.gy0n, fm = c.fm;
+    double aV = 0.0, aCx = 0.0, aCy = 0.0, aPer = 0.0;
+    int blox = dwx, bhix = -1, bloy = dwy, bhiy = -1, pmaxb = 0;
+    bool nonfin = false, clampf = false, ovf = false;
+    for (int i = tid; i < dwx * dwy; i += kNT) {
+        const int b = i / dwx, a = i - (i / dwx) * dwx;
+        const size_t row = (size_t)(dty0 + b) * kTilePitch + dtx0;
+        const float pn = tout[row + a];
+        nonfin |= !isfinite(pn);
+        clampf |= (pn * pn > kQClampPhiSq);
+        pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
+        const uint32_t q = q_of(pn);
+        if (q) {
+            const uint32_t old = atomicAdd(
+                &A.S_sc[(size_t)wrapi(gy0n + b, L) * P + wrapi(gx0n + a, L)], q);
+            ovf |= old > 0xFFFFFFFFu - q;
+        }
+        const double d = (double)pn * (double)pn;
+        aV += d;
+        aCx += d * (double)a;
+        aCy += d * (double)b;
+        if (pn > kSupportEps) {
+            blox = min(blox, a); bhix = max(bhix, a);
+            bloy = min(bloy, b); bhiy = max(bhiy, b);
+        }
+        if (fm) {
+            const float e = a + 1 < dwx ? tout[row + a + 1] : 0.0f;
+            const float w = a > 0 ? tout[row + a - 1] : 0.0f;
+            const float nn = b + 1 < dwy ? tout[row + kTilePitch + a] : 0.0f;
+            const float ss = b > 0 ? tout[row - kTilePitch + a] : 0.0f;
+            const float pgx = 0.5f * (e - w), pgy = 0.5f * (nn - ss);
+            aPer += (double)sqrtf(pgx * pgx + pgy * pgy);
+        }
+    }
+    if (nonfin) flag_or(A.flags, PFH_FLAG_NONFINITE);
+    if (clampf) flag_or(A.flags, PFH_FLAG_Q_CLAMP);
+    if (ovf) flag_or(A.flags, PFH_FLAG_S_OVERFLOW);
+    reduce_commit<CLS>(A, n, sm, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
+                       pmaxb);
+}
+
+// Classes other than the round class 0 run out of line, with their own
+// register allocation, so their longer per-thread columns (up to 29 rows) do
+// not raise the pressure of the hot class-0 path.
+template <int CLS>
+__device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
+                                       unsigned long long step, Smem* sm) {
+    if constexpr (CLS <= kClassStagedMax) cell_staged<CLS>(*A, n, step, *sm);
+    else cell_global<CLS>(*A, n, step, *sm);
+}
+
+// -------------------------------------------------------------------------
+// The persistent step kernel (src/kernels.cu:948-1030).
+// -------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
+    __shared__ Smem sm;
+    const int tid = (int)threadIdx.x;
+
+    // Clear S[(step+2)%3] ahead: this slot is neither read nor scattered
+    // into by this launch. Non-temporal stores keep it out of the caches.
+    if (A.clear_ahead_words > 0ull) {
+        u32x4* dst = reinterpret_cast<u32x4*>(A.S_cl);
+        const u32x4 zero4 = {0u, 0u, 0u, 0u};
+        const size_t nvec = (size_t)(A.clear_ahead_words >> 2);
+        const size_t stride = (size_t)gridDim.x * kNT;
+        for (size_t i = (size_t)blockIdx.x * kNT + tid; i < nvec; i += stride)
+            __builtin_nontemporal_store(zero4, &dst[i]);
+        for (size_t i = (nvec << 2) + (size_t)blockIdx.x * kNT + tid;
+             i < (size_t)A.clear_ahead_words; i += stride)
+            A.S_cl[i] = 0u;
+    }
+    // Cursor and step slots alternate with the phi parity; the kernel
+    // boundary orders the reset and publication for the next launch.
+    if (blockIdx.x == 0 && tid == 0) {
+        *A.cursor_clear = 0ull;
+        *A.step_wr = *A.step_rd + 1ull;
+    }
+    const unsigned long long step = *A.step_rd;
+    if (tid == 0) sm.args = A;
+
+    for (;;) {
+        if (tid == 0) {
+            const unsigned long long k = atomicAdd(A.cursor_use, 1ull);
+            int nn = -1, cl = -1;
+            if (k < (unsigned long long)A.N) {
+                nn = (int)A.perm[k];
+                cl = (int)A.cell_cls[nn];
+            }
+            sm.q_n = nn;
+            sm.q_cls = cl;
+        }
+        __syncthreads();
+        const int cls = sm.q_cls;
+        const int n = __builtin_amdgcn_readfirstlane(sm.q_n);
+        __syncthreads();                  // q_n/q_cls are rewritten next turn
+        if (cls < 0) break;
+        switch (cls) {
+            case 0: cell_staged<0>(A, n, step, sm); break;
+            case 1: cell_cold<1>(&sm.args, n, step, &sm); break;
+            case 2: cell_cold<2>(&sm.args, n, step, &sm); break;
+            case 3: cell_cold<3>(&sm.args, n, step, &sm); break;
+            case 4: cell_cold<4>(&sm.args, n, step, &sm); break;
+            case 5: cell_cold<5>(&sm.args, n, step, &sm); break;
+            default:
+                if (tid == 0) atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
+                break;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// Initial condition, verification, trajectory, ordering.
+// -------------------------------------------------------------------------
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void k_init_tiles(float* phi_a, float* phi_b, pfh_cell_state* cell,
+                             const uint8_t* clsv, int N, int L,
+                             const float* seed_cx, const float* seed_cy,
+                             float radius_eff, float kappa_iface) {
+    const int n = blockIdx.x;
+    if (n >= N) return;
+    const int cls = (int)clsv[n];
+    const Shape sc = shape_of(cls);
+    float* ta = phi_a + (size_t)n * kTileArea;
+    float* tb = phi_b + (size_t)n * kTileArea;
+    for (int i = (int)threadIdx.x; i < kTileArea; i += (int)blockDim.x) {
+        ta[i] = 0.0f;
+        tb[i] = 0.0f;
+    }
+    __syncthreads();
+    const float cx = seed_cx[n], cy = seed_cy[n];
+    const int gx0 = wrapi((int)rintf(cx - 0.5f * (float)(sc.wx - 1)), L);
+    const int gy0 = wrapi((int)rintf(cy - 0.5f * (float)(sc.wy - 1)), L);
+    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += (int)blockDim.x) {
+        const int a = p % sc.wx, b = p / sc.wx;
+        float dx = (float)(gx0 + a) - cx;
+        float dy = (float)(gy0 + b) - cy;
+        if (dx > 0.5f * (float)L) dx -= (float)L;
+        if (dx < -0.5f * (float)L) dx += (float)L;
+        if (dy > 0.5f * (float)L) dy -= (float)L;
+        if (dy < -0.5f * (float)L) dy += (float)L;
+        const float r = sqrtf(dx * dx + dy * dy);
+        const float t = (float)tanh_d((double)(kappa_iface * (r - radius_eff)));
+        ta[(size_t)(sc.ty0 + b) * kTilePitch + sc.tx0 + a] = 0.5f * (1.0f - t);
+    }
+    if (threadIdx.x == 0) {
+        pfh_cell_state* cs = &cell[n];
+        cs->gx0 = gx0;
+        cs->gy0 = gy0;
+        cs->cls = (uint8_t)cls;
+        cs->cls_written[0] = (uint8_t)cls;
+        cs->cls_written[1] = (uint8_t)cls;
+    }
+}
+
+// Moments of the stored field with the update's thread mapping: column bands
+// for the LDS classes, flat pixel index for classes 4-5.
+__global__ __launch_bounds__(kNT) void k_init_moments(pfh_cell_state* cell,
+                                                      const uint8_t* clsv,
+                                                      const float* phi, int N) {
+    __shared__ double red[kWaves * 4];
+    __shared__ int ired[kWaves * 8];
+    const int n = blockIdx.x;
+    if (n >= N) return;
+    const int cls = (int)clsv[n];
+    const Shape sc = shape_of(cls);
+    const int wx = sc.wx, wy = sc.wy;
+    const float* t = phi + (size_t)n * kTileArea + (size_t)sc.ty0 * kTilePitch + sc.tx0;
+    const int tid = (int)threadIdx.x;
+    double aV = 0.0, aCx = 0.0, aCy = 0.0;
+    int blox = wx, bhix = -1, bloy = wy, bhiy = -1, pmaxb = 0;
+    auto visit = [&](int a, int b) {
+        const float pn = t[(size_t)b * kTilePitch + a];
+        const double d = (double)pn * (double)pn;
+        aV += d;
+        aCx += d * (double)a;
+        aCy += d * (double)b;
+        pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
+        if (pn > kSupportEps) {
+            blox = min(blox, a); bhix = max(bhix, a);
+            bloy = min(bloy, b); bhiy = max(bhiy, b);
+        }
+    };
+    if (cls <= kClassStagedMax) {
+        const int nb = kNT / wx, rb = (wy + nb - 1) / nb;
+        const int x = tid % wx, band = tid / wx;
+        if (band < nb)
+            for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);
+    } else {
+        for (int p = tid; p < wx * wy; p += kNT) visit(p % wx, p / wx);
+    }
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
+    aV = wave_sum(aV); aCx = wave_sum(aCx); aCy = wave_sum(aCy);
+    blox = wave_min(blox); bhix = wave_max(bhix);
+    bloy = wave_min(bloy); bhiy = wave_max(bhiy);
+    pmaxb = wave_max(pmaxb);
+    if (lane == 0) {
+        red[wave * 4 + 0] = aV; red[wave * 4 + 1] = aCx; red[wave * 4 + 2] = aCy;
+        ired[wave * 8 + 0] = blox; ired[wave * 8 + 1] = bhix;
+        ired[wave * 8 + 2] = bloy; ired[wave * 8 + 3] = bhiy;
+        ired[wave * 8 + 4] = pmaxb;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double sV = 0.0, sCx = 0.0, sCy = 0.0;
+        int Blox = wx, Bhix = -1, Bloy = wy, Bhiy = -1, Pmax = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            sV += red[w * 4 + 0]; sCx += red[w * 4 + 1]; sCy += red[w * 4 + 2];
+            Blox = min(Blox, ired[w * 8 + 0]); Bhix = max(Bhix, ired[w * 8 + 1]);
+            Bloy = min(Bloy, ired[w * 8 + 2]); Bhiy = max(Bhiy, ired[w * 8 + 3]);
+            Pmax = max(Pmax, ired[w * 8 + 4]);
+        }
+        pfh_cell_state* cs = &cell[n];
+        cs->V = sV; cs->Cx = sCx; cs->Cy = sCy;
+        cs->bb_lo_x = Blox; cs->bb_hi_x = Bhix;
+        cs->bb_lo_y = Bloy; cs->bb_hi_y = Bhiy;
+        cs->phi_max = __uint_as_float((uint32_t)Pmax);
+        cs->promote_ctr = 0u;
+        cs->Ix = 0.0;
+        cs->Iy = 0.0;
+    }
+}
+
+__global__ void k_scatter_all(const float* phi, const pfh_cell_state* cell,
+                              const uint8_t* clsv, uint32_t* S, int N, int L,
+                              int P, uint32_t* flags) {
+    const int n = blockIdx.x;
+    if (n >= N) return;
+    const int cls = (int)clsv[n];
+    const Shape sc = shape_of(cls);
+    const int gx0 = cell[n].gx0, gy0 = cell[n].gy0;
+    const float* t = phi + (size_t)n * kTileArea;
+    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += (int)blockDim.x) {
+        const int a = p % sc.wx, b = p / sc.wx;
+        const float pn = t[(size_t)(sc.ty0 + b) * kTilePitch + sc.tx0 + a];
+        const uint32_t q = q_of(pn);
+        if (!q) continue;
+        const uint32_t old = atomicAdd(
+            &S[(size_t)wrapi(gy0 + b, L) * P + wrapi(gx0 + a, L)], q);
+        if (old > 0xFFFFFFFFu - q) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
+        if (pn * pn > kQClampPhiSq) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_verify_cells(
+    const float* phi, const pfh_cell_state* cell, const uint8_t* clsv, int N,
+    double* out_V, float* out_outside_max) {
+    __shared__ double vred[kWaves];
+    __shared__ float ored[kWaves];
+    const int n = blockIdx.x;
+    if (n >= N) return;
+    const Shape sc = shape_of((int)clsv[n]);
+    const float* t = phi + (size_t)n * kTileArea;
+    double v = 0.0;
+    float omax = 0.0f;
+    for (int i = (int)threadIdx.x; i < kTileArea; i += (int)blockDim.x) {
+        const int x = i % kTilePitch, y = i / kTilePitch;
+        const float p = t[i];
+        const bool inside = x >= sc.tx0 && x < sc.tx0 + sc.wx &&
+                            y >= sc.ty0 && y < sc.ty0 + sc.wy;
+        if (inside) v += (double)p * (double)p;
+        else omax = fmaxf(omax, fabsf(p));
+    }
+    v = wave_sum(v);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        omax = fmaxf(omax, __shfl_down(omax, d, kWave));
+    const int lane = (int)threadIdx.x & (kWave - 1), wave = (int)threadIdx.x / kWave;
+    if (lane == 0) { vred[wave] = v; ored[wave] = omax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sv = 0.0;
+        float so = 0.0f;
+        for (int w = 0; w < (int)blockDim.x / kWave; ++w) {
+            sv += vred[w];
+            so = fmaxf(so, ored[w]);
+        }
+        out_V[n] = sv;
+        out_outside_max[n] = so;
+    }
+}
+
+__global__ void k_verify_S(const uint32_t* S, size_t n, uint32_t* out_max) {
+    uint32_t m = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        m = max(m, S[i]);
+    atomicMax(out_max, m);
+}
+
+__global__ void k_pack_traj(const pfh_cell_state* cell, const uint8_t* clsv,
+                            pfh_traj_packed_cell* out, int N, int L) {
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n >= N) return;
+    const pfh_cell_state cs = cell[n];
+    const double V = cs.V > 0.0 ? cs.V : 1.0;
+    const double cx = (double)cs.gx0 + cs.Cx / V;
+    const double cy = (double)cs.gy0 + cs.Cy / V;
+    pfh_traj_packed_cell t;
+    t.global_id = cs.global_id;
+    t.cls = (int)clsv[n];
+    t.cx = (float)(cx - floor(cx / (double)L) * (double)L);
+    t.cy = (float)(cy - floor(cy / (double)L) * (double)L);
+    t.vx = cs.vx;
+    t.vy = cs.vy;
+    t.theta = cs.theta;
+    t.volume = (float)cs.V;
+    t.perim = (float)cs.perim;
+    t.gamma = cs.gamma;
+    t.v_A = cs.v_A;
+    t.phi_max = cs.phi_max;
+    out[n] = t;
+}
+
+__device__ __forceinline__ uint32_t part1by1(uint32_t x) {
+    x &= 0x0000FFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// One-workgroup bitonic sort of (Morton key of the centroid, index).
+__global__ __launch_bounds__(1024) void k_morton_sort(const pfh_cell_state* cell,
+                                                      uint32_t* perm, int N,
+                                                      int M, int L) {
+    extern __shared__ unsigned long long ks[];
+    for (int i = (int)threadIdx.x; i < M; i += (int)blockDim.x) {
+        unsigned long long k = 0xFFFFFFFFFFFFFFFFull;
+        if (i < N) {
+            const pfh_cell_state cs = cell[i];
+            const double V = cs.V > 0.0 ? cs.V : 1.0;
+            const int cx = wrapi(cs.gx0 + (int)(cs.Cx / V), L);
+            const int cy = wrapi(cs.gy0 + (int)(cs.Cy / V), L);
+            k = ((unsigned long long)(part1by1((uint32_t)cx) |
+                                      (part1by1((uint32_t)cy) << 1)) << 32) |
+                (unsigned long long)(uint32_t)i;
+        }
+        ks[i] = k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= M; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = (int)threadIdx.x; i < M; i += (int)blockDim.x) {
+                const int q = i ^ j;
+                if (q > i) {
+                    const bool up = ((i & k) == 0);
+                    if ((ks[i] > ks[q]) == up) {
+                        const unsigned long long tmp = ks[i];
+                        ks[i] = ks[q];
+                        ks[q] = tmp;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = (int)threadIdx.x; i < N; i += (int)blockDim.x)
+        perm[i] = (uint32_t)(ks[i] & 0xFFFFFFFFull);
+}
+
+}  // namespace pfh
+
+// -------------------------------------------------------------------------
+// C ABI
+// -------------------------------------------------------------------------
+namespace {
+thread_local char g_last_error[256] = {0};
+
+int hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return PFH_OK;
+    snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what,
+             hipGetErrorString(e));
+    return PFH_ERR_HIP;
+}
+int launch_status(const char* what) {
+    return hip_status(hipGetLastError(), what);
+}
+int invalid(const char* what) {
+    snprintf(g_last_error, sizeof(g_last_error), "invalid argument: %s", what);
+    return PFH_ERR_INVALID;
+}
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+}  // namespace
+
+extern "C" {
+
+int pfh_abi_version(void) { return 10000; }
+
+const char* pfh_last_error(void) { return g_last_error; }
+
+int pfh2d_configure_step(void) {
+    // k_step uses static LDS only; nothing to opt into on gfx950.
+    return PFH_OK;
+}
+
+int pfh2d_configure_morton(int smem_bytes) {
+    if (smem_bytes <= 0 || smem_bytes > 160 * 1024) return invalid("morton smem");
+    return hip_status(hipFuncSetAttribute(
+                          reinterpret_cast<const void*>(pfh::k_morton_sort),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, smem_bytes),
+                      "hipFuncSetAttribute(k_morton_sort)");
+}
+
+int pfh2d_step_grid(int device) {
+    int cus = 0;
+    const int st = hip_status(
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+        "hipDeviceGetAttribute(multiProcessorCount)");
+    if (st != PFH_OK) return st;
+    return cus > 0 ? cus : 1;
+}
+
+int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
+                      const void* l2_base, size_t l2_bytes, float l2_hit_ratio,
+                      int observation) {
+    (void)l2_base; (void)l2_bytes; (void)l2_hit_ratio;
+    if (!a) return invalid("args");
+    if (observation != 0) {
+        snprintf(g_last_error, sizeof(g_last_error),
+                 "velocity observation is not part of this build");
+        return PFH_ERR_UNSUPPORTED;
+    }
+    if (a->moments) return invalid("moments must be NULL");
+    if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
+        !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||
+        !a->perm || !a->cursor_use || !a->cursor_clear || !a->step_rd ||
+        !a->step_wr || !a->flags || (a->clear_ahead_words && !a->S_cl) ||
+        a->parity_out < 0 || a->parity_out > 1)
+        return invalid("step args");
+    // Every window must fit strictly inside the periodic domain
+    // (include/params.cuh:666-680), or S self-subtraction is not exact.
+    if (a->L <= 286) return invalid("domain side must exceed 286");
+    hipLaunchKernelGGL(pfh::k_step, dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                       as_stream(stream), *a);
+    return launch_status("k_step");
+}
+
+int pfh2d_query_step_stats(int device, pfh_kernel_stats* out) {
+    if (!out) return invalid("out");
+    hipFuncAttributes fa{};
+    int st = hip_status(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pfh::k_step)),
+                        "hipFuncGetAttributes(k_step)");
+    if (st != PFH_OK) return st;
+    int blocks = 0;
+    st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &blocks, reinterpret_cast<const void*>(pfh::k_step), pfh::kNT, 0),
+                    "hipOccupancyMaxActiveBlocksPerMultiprocessor(k_step)");
+    if (st != PFH_OK) return st;
+    int max_threads = 0;
+    if (hipDeviceGetAttribute(&max_threads, hipDeviceAttributeMaxThreadsPerMultiProcessor,
+                              device) != hipSuccess || max_threads <= 0)
+        max_threads = 2048;
+    memset(out, 0, sizeof(*out));
+    out->regs = fa.numRegs;
+    out->local_bytes = fa.localSizeBytes;
+    out->static_smem = (int)fa.sharedSizeBytes;
+    out->dynamic_smem = 0;
+    out->ctas_per_sm = blocks;
+    out->warps_per_sm = blocks * pfh::kNT / pfh::kWave;
+    out->occupancy = (double)(blocks * pfh::kNT) / (double)max_threads;
+    return PFH_OK;
+}
+
+int pfh2d_init_tiles(float* phi_a, float* phi_b, pfh_cell_state* cell,
+                     const uint8_t* cls, int N, int L, const float* seed_cx,
+                     const float* seed_cy, float radius_eff, float kappa_iface,
+                     void* stream) {
+    if (N < 0 || L <= 0 || (N > 0 && (!phi_a || !phi_b || !cell || !cls ||
+                                      !seed_cx || !seed_cy)))
+        return invalid("init_tiles");
+    if (N == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_init_tiles, dim3(N), dim3(256), 0, as_stream(stream),
+                       phi_a, phi_b, cell, cls, N, L, seed_cx, seed_cy,
+                       radius_eff, kappa_iface);
+    return launch_status("k_init_tiles");
+}
+
+int pfh2d_init_moments(pfh_cell_state* cell, const uint8_t* cls,
+                       const float* phi, int N, void* stream) {
+    if (N < 0 || (N > 0 && (!cell || !cls || !phi))) return invalid("init_moments");
+    if (N == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_init_moments, dim3(N), dim3(pfh::kNT), 0,
+                       as_stream(stream), cell, cls, phi, N);
+    return launch_status("k_init_moments");
+}
+
+int pfh2d_scatter_all(const float* phi, const pfh_cell_state* cell,
+                      const uint8_t* cls, uint32_t* S, int N, int L, int P,
+                      uint32_t* flags, void* stream) {
+    if (N < 0 || L <= 0 || P < L ||
+        (N > 0 && (!phi || !cell || !cls || !S || !flags)))
+        return invalid("scatter_all");
+    if (N == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_scatter_all, dim3(N), dim3(256), 0, as_stream(stream),
+                       phi, cell, cls, S, N, L, P, flags);
+    return launch_status("k_scatter_all");
+}
+
+int pfh2d_zero_u32(uint32_t* p, size_t n, void* stream) {
+    if (n == 0) return PFH_OK;
+    if (!p) return invalid("zero_u32");
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(pfh::k_zero_u32, dim3((unsigned)blocks), dim3(256), 0,
+                       as_stream(stream), p, n);
+    return launch_status("k_zero_u32");
+}
+
+int pfh2d_verify_cells(const float* phi, const pfh_cell_state* cell,
+                       const uint8_t* cls, int N, double* out_V,
+                       float* out_outside_max, void* stream) {
+    if (N < 0 || (N > 0 && (!phi || !cell || !cls || !out_V || !out_outside_max)))
+        return invalid("verify_cells");
+    if (N == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_verify_cells, dim3(N), dim3(1024), 0,
+                       as_stream(stream), phi, cell, cls, N, out_V, out_outside_max);
+    return launch_status("k_verify_cells");
+}
+
+int pfh2d_verify_S(const uint32_t* S, size_t n, uint32_t* out_max, void* stream) {
+    if (!out_max || (n > 0 && !S)) return invalid("verify_S");
+    if (n == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_verify_S, dim3(256), dim3(256), 0, as_stream(stream),
+                       S, n, out_max);
+    return launch_status("k_verify_S");
+}
+
+int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,
+                    pfh_traj_packed_cell* out, int N, int L, void* stream) {
+    if (N < 0 || L <= 0 || (N > 0 && (!cell || !cls || !out))) return invalid("pack_traj");
+    if (N == 0) return PFH_OK;
+    hipLaunchKernelGGL(pfh::k_pack_traj, dim3((N + 127) / 128), dim3(128), 0,
+                       as_stream(stream), cell, cls, out, N, L);
+    return launch_status("k_pack_traj");
+}
+
+int pfh2d_morton_sort(const pfh_cell_state* cell, uint32_t* perm, int N, int M,
+                      int L, void* stream) {
+    if (N <= 0 || M < N || (M & (M - 1)) != 0 || M > 16384 || L <= 0 || !cell || !perm)
+        return invalid("morton_sort");
+    const size_t smem = (size_t)M * sizeof(unsigned long long);
+    if (smem > 48 * 1024) {
+        const int st = pfh2d_configure_morton((int)smem);
+        if (st != PFH_OK) return st;
+    }
+    hipLaunchKernelGGL(pfh::k_morton_sort, dim3(1), dim3(1024), smem,
+                       as_stream(stream), cell, perm, N, M, L);
+    return launch_status("k_morton_sort");
+}
+
+}  // extern "C"

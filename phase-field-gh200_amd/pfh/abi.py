@@ -1,0 +1,138 @@
+"""ctypes view of the C ABI in include/pfh.h.
+
+Loads lib/libpfh.so (built by build.py) and declares every entry point and
+POD record. Raises ImportError when the library is missing: there is no
+fallback implementation of any kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libpfh.so"
+
+TILE_PITCH = 288
+TILE_AREA = TILE_PITCH * TILE_PITCH
+NUM_CLASSES = 6
+CLASSES = [(144, 144, 64, 64), (176, 144, 32, 64), (144, 176, 64, 32),
+           (160, 160, 32, 32), (224, 224, 32, 32), (286, 286, 1, 1)]
+
+FLAG_NAMES = ["S_overflow", "q_clamp", "support_clip", "class_exhausted",
+              "S_other_negative", "phi_nonfinite", "V_nonpositive",
+              "class_unsupported"]
+FLAG_COUNT = 8
+FLAG_SUPPORT_CLIP = 2
+
+PFH_OK = 0
+
+# CellState (include/kernels.cuh:27-57): 192 bytes.
+CELL_DTYPE = np.dtype({
+    "names": ["gx0", "gy0", "cls", "cls_written", "pad0", "global_id",
+              "gamma", "v_A", "R_tgt", "theta", "V", "Cx", "Cy", "perim",
+              "bb_lo_x", "bb_hi_x", "bb_lo_y", "bb_hi_y", "vx", "vy", "Ix",
+              "Iy", "promote_ctr", "shift_ctr", "tumble_ctr", "phi_max",
+              "reserved", "adv_q", "adv_count", "reserved_unused", "M_pf"],
+    "formats": ["<i4", "<i4", "u1", ("u1", (2,)), "u1", "<i4", "<f4", "<f4",
+                "<f4", "<f4", "<f8", "<f8", "<f8", "<f8", "<i4", "<i4", "<i4",
+                "<i4", "<f4", "<f4", "<f8", "<f8", "<u4", "<u4", "<u4", "<f4",
+                ("<u4", (2,)), ("<f8", (6,)), "<u8", "<u4", "<f4"],
+    "offsets": [0, 4, 8, 9, 11, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 68,
+                72, 76, 80, 84, 88, 96, 104, 108, 112, 116, 120, 128, 176,
+                184, 188],
+    "itemsize": 192,
+})
+
+# TrajPackedCell (include/kernels.cuh:60-70): 48 bytes.
+TRAJ_DTYPE = np.dtype([("global_id", "<i4"), ("cls", "<i4"), ("cx", "<f4"),
+                       ("cy", "<f4"), ("vx", "<f4"), ("vy", "<f4"),
+                       ("theta", "<f4"), ("volume", "<f4"), ("perim", "<f4"),
+                       ("gamma", "<f4"), ("v_A", "<f4"), ("phi_max", "<f4")])
+
+
+class StepArgs(C.Structure):
+    """pfh2d_step_args (mirror of StepArgs, include/kernels.cuh:74-116)."""
+    _fields_ = [
+        ("phi_in", C.c_void_p), ("phi_out", C.c_void_p),
+        ("S_rd", C.c_void_p), ("S_sc", C.c_void_p), ("S_cl", C.c_void_p),
+        ("cell", C.c_void_p), ("cell_cls", C.c_void_p), ("perm", C.c_void_p),
+        ("cursor_use", C.c_void_p), ("cursor_clear", C.c_void_p),
+        ("step_rd", C.c_void_p), ("step_wr", C.c_void_p),
+        ("flags", C.c_void_p), ("moments", C.c_void_p),
+        ("physical_dt", C.c_double),
+        ("N", C.c_int), ("L", C.c_int), ("P", C.c_int), ("parity_out", C.c_int),
+        ("dt", C.c_float), ("A0", C.c_double), ("vol_scale", C.c_double),
+        ("bulk_scale", C.c_float), ("rep_coeff", C.c_float),
+        ("mot_coeff", C.c_float),
+        ("seed", C.c_ulonglong), ("polarity_seed", C.c_ulonglong),
+        ("p_tumble", C.c_double), ("full_moment_every", C.c_int),
+        ("clear_ahead_words", C.c_ulonglong),
+    ]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("regs", C.c_int), ("agprs", C.c_int), ("sgprs", C.c_int),
+                ("local_bytes", C.c_size_t), ("static_smem", C.c_int),
+                ("dynamic_smem", C.c_int), ("ctas_per_sm", C.c_int),
+                ("warps_per_sm", C.c_int), ("occupancy", C.c_double)]
+
+
+assert C.sizeof(StepArgs) == 216, C.sizeof(StepArgs)
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_SZ = C.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pfh_abi_version": (_I, []),
+    "pfh_last_error": (C.c_char_p, []),
+    "pfh2d_configure_step": (_I, []),
+    "pfh2d_configure_morton": (_I, [_I]),
+    "pfh2d_step_grid": (_I, [_I]),
+    "pfh2d_launch_step": (_I, [C.POINTER(StepArgs), _I, _P, _P, _SZ, _F, _I]),
+    "pfh2d_query_step_stats": (_I, [_I, C.POINTER(KernelStats)]),
+    "pfh2d_init_tiles": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _F, _F, _P]),
+    "pfh2d_init_moments": (_I, [_P, _P, _P, _I, _P]),
+    "pfh2d_scatter_all": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P]),
+    "pfh2d_zero_u32": (_I, [_P, _SZ, _P]),
+    "pfh2d_verify_cells": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+    "pfh2d_verify_S": (_I, [_P, _SZ, _P, _P]),
+    "pfh2d_pack_traj": (_I, [_P, _P, _P, _I, _I, _P]),
+    "pfh2d_morton_sort": (_I, [_P, _P, _I, _I, _I, _P]),
+}
+
+# Entry points the ABI header declares; tests check the library exports all.
+EXPORTED = list(SIGNATURES)
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load lib/libpfh.so once. Raises ImportError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python phase-field-gh200_amd/build.py` (there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PfhError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str) -> None:
+    if status != PFH_OK:
+        msg = load().pfh_last_error().decode(errors="replace")
+        raise PfhError(f"{what} failed with status {status}: {msg}")

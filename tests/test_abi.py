@@ -1,0 +1,74 @@
+"""The C-ABI library loads, exports every entry point include/pfh.h declares,
+keeps the reference's record layouts, and rejects bad arguments before any
+GPU work (no device is needed for any of this)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pfh import abi
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "pfh.h"
+
+
+def declared_functions() -> list[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pfh\w*)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    names = declared_functions()
+    assert names, "no declarations parsed"
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(abi.EXPORTED) == names
+
+
+def test_abi_version():
+    assert abi.load().pfh_abi_version() == 10000
+
+
+def test_cell_state_layout_matches_reference():
+    # include/kernels.cuh:53-57 pins these; CellState is 192 B.
+    d = abi.CELL_DTYPE
+    assert d.itemsize == 192
+    assert d.fields["advection" if "advection" in d.fields else "adv_q"][1] == 128
+    assert d.fields["reserved"][1] == 120
+    assert d.fields["M_pf"][1] == 188
+    assert abi.TRAJ_DTYPE.itemsize == 48
+    assert C.sizeof(abi.StepArgs) == 216
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    lib = abi.load()
+    assert lib.pfh2d_launch_step(None, 1, None, None, 0, 0.0, 0) == -1
+    a = abi.StepArgs()
+    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 0) == -1
+    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 1) == -3   # observer
+    assert b"observation" in lib.pfh_last_error()
+    assert lib.pfh2d_init_tiles(None, None, None, None, 4, 64, None, None, 1.0, 1.0, None) == -1
+    assert lib.pfh2d_init_moments(None, None, None, -1, None) == -1
+    assert lib.pfh2d_scatter_all(None, None, None, None, 2, 100, 50, None, None) == -1
+    assert lib.pfh2d_morton_sort(None, None, 5, 6, 64, None) == -1      # M not a power of 2
+    assert lib.pfh2d_verify_S(None, 4, None, None) == -1
+    assert lib.pfh2d_pack_traj(None, None, None, 3, 0, None) == -1
+    # zero-sized requests are no-ops that touch nothing
+    assert lib.pfh2d_init_tiles(None, None, None, None, 0, 64, None, None, 1.0, 1.0, None) == 0
+    assert lib.pfh2d_zero_u32(None, 0, None) == 0
+
+
+def test_launch_rejects_small_domain():
+    lib = abi.load()
+    a = abi.StepArgs()
+    dummy = np.zeros(16, np.uint64)
+    for f in ("phi_in", "phi_out", "S_rd", "S_sc", "S_cl", "cell", "cell_cls", "perm",
+              "cursor_use", "cursor_clear", "step_rd", "step_wr", "flags"):
+        setattr(a, f, dummy.ctypes.data)
+    a.N, a.L, a.P = 4, 256, 256
+    # L <= 286: a window would alias onto itself (include/params.cuh:666-680)
+    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 0) == -1
+    assert b"286" in lib.pfh_last_error()
