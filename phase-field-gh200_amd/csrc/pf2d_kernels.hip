@@ -780,18 +780,27 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
     if (tid == 0) sm.args = A;
 
     for (;;) {
+        // Thread 0 takes the ticket and also flags an unsupported class id
+        // here, so that every branch after the broadcast is workgroup-uniform
+        // with no single-thread work: a thread-0-only statement in the
+        // dispatch lets the control-flow structurizer fold it into the loop
+        // head, where the other lanes would re-read a stale ticket forever.
         if (tid == 0) {
             const unsigned long long k = atomicAdd(A.cursor_use, 1ull);
             int nn = -1, cl = -1;
             if (k < (unsigned long long)A.N) {
                 nn = (int)A.perm[k];
                 cl = (int)A.cell_cls[nn];
+                if (cl >= kNumClasses) {
+                    atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
+                    cl = kNumClasses;        // skipped below
+                }
             }
             sm.q_n = nn;
             sm.q_cls = cl;
         }
         __syncthreads();
-        const int cls = sm.q_cls;
+        const int cls = __builtin_amdgcn_readfirstlane(sm.q_cls);
         const int n = __builtin_amdgcn_readfirstlane(sm.q_n);
         __syncthreads();                  // q_n/q_cls are rewritten next turn
         if (cls < 0) break;
@@ -802,9 +811,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
             case 3: cell_cold<3>(&sm.args, n, step, &sm); break;
             case 4: cell_cold<4>(&sm.args, n, step, &sm); break;
             case 5: cell_cold<5>(&sm.args, n, step, &sm); break;
-            default:
-                if (tid == 0) atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
-                break;
+            default: break;                  // unsupported: flagged above
         }
     }
 }
