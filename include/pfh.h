@@ -164,6 +164,11 @@ int pfh_abi_version(void);
 /* Last HIP error text recorded by this thread's failing call ("" if none). */
 const char* pfh_last_error(void);
 
+/* Plain 16-byte-vector device copy of `bytes` (a multiple of 16, 16 B
+ * aligned). No counterpart in the reference: it is the known-size kernel that
+ * calibrates hardware traffic counters in the measurement tooling. */
+int pfh_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
+
 /* ---- 2D launch API ---- */
 
 /* Replaces configure_k_step_smem() (include/kernels.cuh:219): sets the

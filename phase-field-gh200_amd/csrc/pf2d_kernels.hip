@@ -992,6 +992,15 @@ __global__ __launch_bounds__(1024) void k_verify_cells(
     }
 }
 
+// Plain streaming copy: the known-size reference kernel used to calibrate
+// hardware byte counters against bytes actually moved.
+__global__ __launch_bounds__(256) void k_stream_copy(u32x4* dst, const u32x4* src,
+                                                     size_t nvec) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_verify_S(const uint32_t* S, size_t n, uint32_t* out_max) {
     uint32_t m = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1237,6 +1246,19 @@ int pfh2d_verify_S(const uint32_t* S, size_t n, uint32_t* out_max, void* stream)
     hipLaunchKernelGGL(pfh::k_verify_S, dim3(256), dim3(256), 0, as_stream(stream),
                        S, n, out_max);
     return launch_status("k_verify_S");
+}
+
+int pfh_stream_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return PFH_OK;
+    if (!dst || !src || (bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u))
+        return invalid("stream_copy needs 16 B aligned pointers and sizes");
+    const size_t nvec = bytes / 16;
+    size_t blocks = (nvec + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(pfh::k_stream_copy, dim3((unsigned)blocks), dim3(256), 0,
+                       as_stream(stream), reinterpret_cast<pfh::u32x4*>(dst),
+                       reinterpret_cast<const pfh::u32x4*>(src), nvec);
+    return launch_status("k_stream_copy");
 }
 
 int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,

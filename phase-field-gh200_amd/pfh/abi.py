@@ -89,6 +89,7 @@ _SZ = C.c_size_t
 SIGNATURES = {
     "pfh_abi_version": (_I, []),
     "pfh_last_error": (C.c_char_p, []),
+    "pfh_stream_copy": (_I, [_P, _P, _SZ, _P]),
     "pfh2d_configure_step": (_I, []),
     "pfh2d_configure_morton": (_I, [_I]),
     "pfh2d_step_grid": (_I, [_I]),
