@@ -43,14 +43,27 @@ struct Ctl {
     float gam, dwC, volC, vA, cosp, sinp, theta, mrel, vx, vy;
 };
 
+// The cell a workgroup processes next: filled by the control wavefront
+// during the previous cell, double-buffered.
+struct Slot {
+    int n, cls;
+    Ctl ctl;
+};
+
 struct Smem {
     pfh2d_step_args args;        // LDS copy for the out-of-line class paths
     double red[kWaves * 8];
     int ired[kWaves * 8];
-    Ctl ctl;
-    int q_n, q_cls;
-    float phi[kPhiLdsFloats];
+    Slot slot[2];
+    uint32_t csraw[2][48];       // CellState (192 B) of the next cell
+    alignas(16) float phi[kPhiLdsFloats];
 };
+
+// Control lanes: the last 16 lanes of the workgroup. No LDS-path class gives
+// them pixels (7x144, 5x176, 7x144 and 6x160 threads are at most 1008), so
+// they fetch the next cell while the other lanes compute.
+constexpr int kControlTid0 = kNT - 16;
+constexpr int kLeaderLane = kControlTid0 % kWave;     // lane 48 of wave 15
 
 // threadIdx.x made opaque inside the persistent loop: without this, LICM
 // hoists every per-thread index and address of the cell body out of the loop
@@ -108,14 +121,13 @@ __device__ __forceinline__ ScalarArgs scalar_args(const pfh2d_step_args& A) {
 }
 
 // Per-cell scalars: tumble, polarity, coefficients, class hysteresis and the
-// recentring shift (src/kernels.cu:93-191 and :624-711).
-// Out of line: its double-precision polynomial constants must not be hoisted
-// out of the persistent loop into the update's register budget.
-template <int CLS>
-__device__ __noinline__ void cell_scalars(const ScalarArgs A, int n,
-                                         unsigned long long step, Ctl* ctl) {
-    constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
-    const pfh_cell_state& cs = A.cell[n];
+// recentring shift (src/kernels.cu:93-191 and :624-711), from a copy of the
+// cell's state. Out of line: its double-precision polynomial constants must
+// not be hoisted out of the persistent loop into the update's registers.
+__device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_state* csp,
+                                         int CLS, unsigned long long step, Ctl* ctl) {
+    const int WX = shape_of(CLS).wx, WY = shape_of(CLS).wy;
+    const pfh_cell_state& cs = *csp;
     const Philox4 r = philox4x32_10(
         (uint32_t)(step & 0xFFFFFFFFull), (uint32_t)(step >> 32),
         (uint32_t)cs.global_id, 0xA5A5A5A5u,
@@ -218,7 +230,7 @@ __device__ void zero_outside_window(float* tile, int dwx, int dwy, int dtx0,
 // Velocity from the interaction integral (src/kernels.cu:300-342): wave
 // trees, then thread 0 adds the 16 wave sums in order and commits.
 __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
-                                double aIx, double aIy) {
+                                Ctl& c, double aIx, double aIy) {
     const int tid = opaque_tid();
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -235,12 +247,12 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
             sIx += sm.red[w * 8 + 0];
             sIy += sm.red[w * 8 + 1];
         }
-        const double vxd = (double)sm.ctl.vA * (double)sm.ctl.cosp
+        const double vxd = (double)c.vA * (double)c.cosp
                          + (double)A.mot_coeff * sIx;
-        const double vyd = (double)sm.ctl.vA * (double)sm.ctl.sinp
+        const double vyd = (double)c.vA * (double)c.sinp
                          + (double)A.mot_coeff * sIy;
-        sm.ctl.vx = (float)vxd;
-        sm.ctl.vy = (float)vyd;
+        c.vx = (float)vxd;
+        c.vy = (float)vyd;
         pfh_cell_state* cs = &A.cell[n];
         cs->vx = (float)vxd;
         cs->vy = (float)vyd;
@@ -253,7 +265,7 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
 // Moments/support reduction and CellState commit (src/kernels.cu:542-591).
 template <int CLS>
 __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
-                              double aV, double aCx, double aCy, double aPer,
+                              const Ctl& c, double aV, double aCx, double aCy, double aPer,
                               int blox, int bhix, int bloy, int bhiy,
                               int pmaxb) {
     const int tid = opaque_tid();
@@ -276,7 +288,6 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const Ctl& c = sm.ctl;
         double sV = 0.0, sCx = 0.0, sCy = 0.0, sPer = 0.0;
         int Blox = c.dwx, Bhix = -1, Bloy = c.dwy, Bhiy = -1, Pmax = 0;
         for (int w = 0; w < kWaves; ++w) {
@@ -318,66 +329,146 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
 }
 
 // -------------------------------------------------------------------------
-// LDS path: classes 0-3.
+// Next-cell fetch by the control lanes, in four stages so that each stage
+// only waits on a load issued one phase earlier:
+//   1 the queue ticket (atomic on the cursor);
+//   2 the cell id perm[ticket];
+//   3 its class and its 192-byte CellState (3 dwords per control lane);
+//   4 the CellState copy into LDS, the slot, and the per-cell scalars.
+// An unsupported class id is flagged here (reference dispatch default,
+// src/kernels.cu:1017-1021) and skipped by the dispatch.
 // -------------------------------------------------------------------------
+struct Fetch {
+    unsigned long long tk;
+    int nn, cls;
+    uint32_t w0, w1, w2;
+};
 
-// Column walk over LDS rows: thread (x, band) visits rows y0..y0+RB-1 of
-// column x with a rolling 3x3 neighbourhood (three LDS reads per row).
+__device__ __forceinline__ void fetch1(const pfh2d_step_args& A, Fetch& f) {
+    if (((int)threadIdx.x & (kWave - 1)) == kLeaderLane)
+        f.tk = atomicAdd(A.cursor_use, 1ull);
+}
+
+__device__ __forceinline__ void fetch2(const pfh2d_step_args& A, Fetch& f) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)f.tk, kLeaderLane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(f.tk >> 32), kLeaderLane);
+    const unsigned long long k = ((unsigned long long)hi << 32) | lo;
+    f.nn = k < (unsigned long long)A.N ? (int)A.perm[k] : -1;
+}
+
+__device__ __forceinline__ void fetch3(const pfh2d_step_args& A, Fetch& f) {
+    if (f.nn >= 0) {
+        const int j = ((int)threadIdx.x & (kWave - 1)) - kLeaderLane;
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(A.cell + f.nn);
+        f.cls = (int)A.cell_cls[f.nn];
+        f.w0 = w[j];
+        f.w1 = w[j + 16];
+        f.w2 = w[j + 32];
+    }
+}
+
+__device__ __forceinline__ void fetch4(const pfh2d_step_args& A, Smem& sm, Fetch& f,
+                                       int which, unsigned long long step) {
+    const int j = ((int)threadIdx.x & (kWave - 1)) - kLeaderLane;
+    int cls = -1;
+    if (f.nn >= 0) {
+        sm.csraw[which][j] = f.w0;
+        sm.csraw[which][j + 16] = f.w1;
+        sm.csraw[which][j + 32] = f.w2;
+        cls = f.cls < kNumClasses ? f.cls : kNumClasses;
+    }
+    if (j == 0) {
+        if (cls == kNumClasses) atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
+        sm.slot[which].n = f.nn;
+        sm.slot[which].cls = cls;
+        if (cls >= 0 && cls < kNumClasses)
+            cell_scalars(scalar_args(A),
+                         reinterpret_cast<const pfh_cell_state*>(sm.csraw[which]), cls,
+                         step, &sm.slot[which].ctl);
+    }
+}
+
+__device__ __forceinline__ void fetch_all(const pfh2d_step_args& A, Smem& sm, int which,
+                                          unsigned long long step) {
+    Fetch f;
+    fetch1(A, f);
+    fetch2(A, f);
+    fetch3(A, f);
+    fetch4(A, sm, f, which, step);
+}
+
+// -------------------------------------------------------------------------
+// LDS path: classes 0-3.
+//
+// LDS holds the window with a one-pixel zero ring: row r (= y + 1, y in
+// [-1, WY]) starts at r*PX with four pad floats (the last one is column -1)
+// and column WX falls on the next row's first pad, so the stencil never needs
+// an edge test. Thread (band, x) walks rows y0..y0+RB-1 of column x with a
+// rolling 3x3 neighbourhood: three LDS reads per row.
+//
+// Exact shortcuts (phi never takes the value -0, so adding or subtracting a
+// signed zero changes no sum or right-hand side):
+//   - a wave-row where no lane overlaps another cell (S_other == 0) skips the
+//     repulsion term and the fp64 interaction-integral terms, all zero;
+//   - a wave-row where every lane has c == passive == gx == gy == 0 has
+//     phi_next == +0 exactly and only stores the zeros.
+// -------------------------------------------------------------------------
 struct Roll {
     float sW, sC, sE, cW, cC, cE, nW, nC, nE;
 };
 
-template <int WX>
-__device__ __forceinline__ void roll_row(const float* col, int r, bool hasW,
-                                         bool hasE, float& w, float& c, float& e) {
-    const float* p = col + r * WX;
+__device__ __forceinline__ void load_row(const float* p, float& w, float& c, float& e) {
+    w = p[-1];
     c = p[0];
-    const float wr = p[-1], er = p[1];
-    w = hasW ? wr : 0.0f;
-    e = hasE ? er : 0.0f;
+    e = p[1];
 }
 
-// Passive right-hand side of phase_field_rhs (include/kernels.cuh:127-136),
-// source order, no contraction. Also returns S_other via So.
-__device__ __forceinline__ float passive_rhs(const Roll& q, uint32_t qS,
-                                             float gam, float dwC, float volC,
-                                             float repC, float& So,
-                                             uint32_t& negc) {
+__device__ __forceinline__ bool wave_any(bool v) {
+    return __builtin_amdgcn_ballot_w64(v) != 0ull;
+}
+
+// First part of phase_field_rhs (include/kernels.cuh:127-136): every passive
+// term but the repulsion, in source order.
+__device__ __forceinline__ float passive_head(const Roll& q, float gam, float dwC,
+                                              float volC) {
     const float lap = (4.0f * (q.nC + q.sC + q.cE + q.cW)
                        + 1.0f * (q.nE + q.nW + q.sE + q.sW)
                        + -20.0f * q.cC) * (float)(1.0 / 6.0);
-    const uint32_t qc = q_of(q.cC);
-    So = 0.0f;
-    if (qS >= qc) So = (float)(qS - qc) * kQInv;
-    else negc += 2u;     // the reference evaluates S_other once per pass
     return gam * lap
          - dwC * (q.cC * (1.0f - q.cC) * (1.0f - 2.0f * q.cC))
-         + volC * q.cC
-         - repC * q.cC * So;
+         + volC * q.cC;
+}
+
+__device__ __forceinline__ float s_other_of(uint32_t qS, float c, uint32_t& negc) {
+    const uint32_t qc = q_of(c);
+    const bool neg = qS < qc;
+    negc += neg ? 2u : 0u;          // the reference evaluates S_other once per pass
+    return neg ? 0.0f : (float)(qS - qc) * kQInv;
 }
 
 template <int CLS>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
-                            unsigned long long step, Smem& sm) {
+                            unsigned long long step, Smem& sm, Ctl& c, int nxt) {
     using G = Staged<CLS>;
     constexpr int WX = G::WX, WY = G::WY, TX0 = G::TX0, TY0 = G::TY0;
+    constexpr int PX = G::PX;
     constexpr int RB = G::RB, NCH = G::NCH, NCHUNK = G::NCHUNK, KCH = G::KCH;
+    constexpr bool kExact = G::NB * RB == WY;   // no partial band
     const int tid = opaque_tid();
     const int x = tid % WX;
     const int band = tid / WX;
     const bool active = band < G::NB;
+    const bool control = tid >= kControlTid0;
     const int y0 = band * RB;
+    Fetch nf;
+    if (control) fetch1(A, nf);
     const int L = A.L, P = A.P;
     const int gx0 = A.cell[n].gx0;
     const int gy0 = A.cell[n].gy0;
-    float* phi_s = sm.phi + 4;   // 4-float guard keeps x-1 / x+1 in bounds
-    const float* col = phi_s + x;
-    const bool hasW = x > 0, hasE = x < WX - 1;
+    float* phi_s = sm.phi;
+    const float* col = phi_s + 4 + x;           // row r of column x: col[r*PX]
     const int gx = (gx0 + x < L) ? gx0 + x : gx0 + x - L;
-
-    // Thread 0 first, while nothing else is live across its call; the other
-    // fifteen wavefronts go straight on to issue their loads.
-    if (tid == 0) cell_scalars<CLS>(scalar_args(A), n, step, &sm.ctl);
+    auto row_ok = [&](int k) { return kExact || y0 + k < WY; };
 
     // Window rows of phi in flight (16 B each) ...
     const PFH_GLOBAL float* tin = gptr(A.phi_in) + (size_t)n * kTileArea;
@@ -396,10 +487,9 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     auto load_qs = [&](uint32_t (&qS)[RB]) {
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
-            const int y = y0 + k;
             qS[k] = 0u;
-            if (active && y < WY) {
-                int gy = gy0 + y;
+            if (active && row_ok(k)) {
+                int gy = gy0 + y0 + k;
                 if (gy >= L) gy -= L;
                 qS[k] = S_rd[(size_t)gy * P + gx];
             }
@@ -412,16 +502,20 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         const int u = tid + k * kNT;
         if (u < NCHUNK) {
             const int r = u / NCH, c4 = u - (u / NCH) * NCH;
-            *reinterpret_cast<f32x4*>(phi_s + (r + 1) * WX + 4 * c4) = ph[k];
+            *reinterpret_cast<f32x4*>(phi_s + (r + 1) * PX + 4 + 4 * c4) = ph[k];
         }
     }
-    for (int i = tid; i < WX; i += kNT) {
-        phi_s[i] = 0.0f;                     // row -1
-        phi_s[(WY + 1) * WX + i] = 0.0f;     // row WY
+    // zero ring: the pads of every row (column -1 and column WX) and rows -1, WY
+    for (int i = tid; i < (WY + 3) * 4 + 2 * WX; i += kNT) {
+        if (i < (WY + 3) * 4) phi_s[(i >> 2) * PX + (i & 3)] = 0.0f;
+        else {
+            const int j = i - (WY + 3) * 4;
+            phi_s[(j < WX ? 0 : (WY + 1) * PX - WX) + 4 + j] = 0.0f;
+        }
     }
     __syncthreads();
+    if (control) fetch2(A, nf);
 
-    const Ctl& c = sm.ctl;
     const float gam = c.gam, dwC = c.dwC, volC = c.volC;
     const float repC = A.rep_coeff;
     if (c.zero_tile)
@@ -434,19 +528,24 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     uint32_t negc = 0u;
     if (active) {
         Roll q;
-        roll_row<WX>(col, y0, hasW, hasE, q.sW, q.sC, q.sE);       // row y0-1
-        roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);   // row y0
+        load_row(col + y0 * PX, q.sW, q.sC, q.sE);          // row y0 - 1
+        load_row(col + (y0 + 1) * PX, q.cW, q.cC, q.cE);    // row y0
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             pas[k] = 0.0f;
-            if (y0 + k < WY) {
-                roll_row<WX>(col, y0 + k + 2, hasW, hasE, q.nW, q.nC, q.nE);
-                float So;
-                pas[k] = passive_rhs(q, qS[k], gam, dwC, volC, repC, So, negc);
-                const float gxv = 0.5f * (q.cE - q.cW);
-                const float gyv = 0.5f * (q.nC - q.sC);
-                aIx += (double)(q.cC * gxv * So);
-                aIy += (double)(q.cC * gyv * So);
+            if (row_ok(k)) {
+                load_row(col + (y0 + k + 2) * PX, q.nW, q.nC, q.nE);
+                const float head = passive_head(q, gam, dwC, volC);
+                const float So = s_other_of(qS[k], q.cC, negc);
+                if (wave_any(So != 0.0f)) {
+                    pas[k] = head - repC * q.cC * So;
+                    const float gxv = 0.5f * (q.cE - q.cW);
+                    const float gyv = 0.5f * (q.nC - q.sC);
+                    aIx += (double)(q.cC * gxv * So);
+                    aIy += (double)(q.cC * gyv * So);
+                } else {
+                    pas[k] = head;
+                }
                 q.sW = q.cW; q.sC = q.cC; q.sE = q.cE;
                 q.cW = q.nW; q.cC = q.nC; q.cE = q.nE;
             }
@@ -457,7 +556,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     }
     if (negc) flag_add(A.flags, PFH_FLAG_S_NEGATIVE, negc);
 
-    reduce_velocity(A, n, sm, aIx, aIy);
+    reduce_velocity(A, n, sm, c, aIx, aIy);
+    if (control) fetch3(A, nf);
 
     // ---- phase B: Euler update, shifted store, scatter, moments ----
     const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
@@ -467,55 +567,58 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     double aV = 0.0, aCx = 0.0, aCy = 0.0, aPer = 0.0;
     int blox = dwx, bhix = -1, bloy = dwy, bhiy = -1, pmaxb = 0;
     bool nonfin = false, clampf = false, ovf = false;
-    auto update = [&](const Roll& q, float pas_k) -> float {
-        const float gxv = 0.5f * (q.cE - q.cW);
-        const float gyv = 0.5f * (q.nC - q.sC);
+    auto update = [&](float cC, float gxv, float gyv, float pas_k) -> float {
         const float adv = vx * gxv + vy * gyv;
         const float rhs = (mrel == 1.0f) ? pas_k - adv : mrel * pas_k - adv;
-        return q.cC + dt * rhs;
+        return cC + dt * rhs;
     };
+    const int a = x - sx;
     if (active) {
-        const int a = x - sx;
         const bool ain = (unsigned)a < (unsigned)dwx;
         uint32_t olds[RB];
-        Roll q;
-        q.sW = q.sE = q.nW = q.nE = 0.0f;
-        q.sC = col[y0 * WX];
-        roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);
+        int rlo = dwy, rhi = -1;
+        float sC = col[y0 * PX];
+        float cW, cC, cE;
+        load_row(col + (y0 + 1) * PX, cW, cC, cE);
+        PFH_GLOBAL float* orow = tout + (ptrdiff_t)(dty0 + y0 - sy) * kTilePitch + dtx0 + a;
+        int gy = gy0 + y0;
+        if (gy >= L) gy -= L;
+        const double ad = (double)a;
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             const int y = y0 + k;
             olds[k] = 0u;
-            if (y < WY) {
-                float nW, nE;
-                roll_row<WX>(col, y + 2, hasW, hasE, nW, q.nC, nE);
-                const float pn = update(q, pas[k]);
+            if (row_ok(k)) {
+                float nW, nC, nE;
+                load_row(col + (y + 2) * PX, nW, nC, nE);
+                const float gxv = 0.5f * (cE - cW);
+                const float gyv = 0.5f * (nC - sC);
                 const int b = y - sy;
-                if (ain && (unsigned)b < (unsigned)dwy) {
-                    tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = pn;
-                    nonfin |= !isfinite(pn);
-                    clampf |= (pn * pn > kQClampPhiSq);
-                    pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
-                    const uint32_t qv = q_of(pn);
-                    if (qv) {
-                        int gy = gy0 + y;
-                        if (gy >= L) gy -= L;
-                        olds[k] = atomic_add_g(&A.S_sc[(size_t)gy * P + gx], qv);
-                    }
-                    const double d = (double)pn * (double)pn;
-                    aV += d;
-                    aCx += d * (double)a;
-                    aCy += d * (double)b;
-                    if (pn > kSupportEps) {
-                        blox = min(blox, a); bhix = max(bhix, a);
-                        bloy = min(bloy, b); bhiy = max(bhiy, b);
-                    }
-                }
-                q.sC = q.cC;
-                q.cW = nW; q.cC = q.nC; q.cE = nE;
+                const bool valid = ain && (unsigned)b < (unsigned)dwy;
+                const float pn = update(cC, gxv, gyv, pas[k]);
+                if (valid) *orow = pn;
+                // A lane whose destination is outside the window contributes
+                // an exact zero to every sum below.
+                const float pv = valid ? pn : 0.0f;
+                const float sq = pv * pv;
+                nonfin |= !isfinite(pv);
+                clampf |= (sq > kQClampPhiSq);
+                pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pv)));
+                const uint32_t qv = __float2uint_rn(fminf(sq, kQClampPhiSq) * kQScale);
+                if (qv) olds[k] = atomic_add_g(&A.S_sc[(size_t)gy * P + gx], qv);
+                const double d = (double)pv * (double)pv;
+                aV += d;
+                aCy += d * (double)b;
+                if (pv > kSupportEps) { rlo = min(rlo, b); rhi = max(rhi, b); }
+                sC = cC;
+                cW = nW; cC = nC; cE = nE;
+                orow += kTilePitch;
+                if (++gy == L) gy = 0;
             }
             __builtin_amdgcn_sched_barrier(0);
         }
+        aCx = aV * ad;                 // every pixel of this thread has column a
+        if (rhi >= 0) { blox = a; bhix = a; bloy = rlo; bhiy = rhi; }
         // Exact S_overflow test with one register per pixel: every addend is
         // at most 2^29, so a wrap needs old >= 2^32 - 2^29; only such
         // candidates recompute q from this thread's own stored value.
@@ -523,8 +626,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         for (int k = 0; k < RB; ++k) {
             if (olds[k] >= 0xE0000000u) {
                 const int b = y0 + k - sy;
-                const uint32_t qv =
-                    q_of(tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a]);
+                const uint32_t qv = q_of(tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a]);
                 ovf |= olds[k] > 0xFFFFFFFFu - qv;
             }
         }
@@ -538,24 +640,24 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         const int ny = sy > 0 ? sy : -sy, nx = sx > 0 ? sx : -sx;
         const int rows = ny * dwx, total = rows + nx * dwy;
         for (int i = tid; i < total; i += kNT) {
-            int a, b;
+            int aa, b;
             if (i < rows) {
                 const int rr = i / dwx;
-                a = i - rr * dwx;
+                aa = i - rr * dwx;
                 b = sy > 0 ? dwy - sy + rr : rr;
             } else {
                 const int j = i - rows, cc = j / dwy;
                 b = j - cc * dwy;
-                a = sx > 0 ? dwx - sx + cc : cc;
+                aa = sx > 0 ? dwx - sx + cc : cc;
             }
-            tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = 0.0f;
+            tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + aa] = 0.0f;
         }
     } else {
         for (int i = tid; i < dwx * dwy; i += kNT) {
-            const int b = i / dwx, a = i - (i / dwx) * dwx;
-            const int ii = a + sx, jj = b + sy;
+            const int b = i / dwx, aa = i - (i / dwx) * dwx;
+            const int ii = aa + sx, jj = b + sy;
             if (!((unsigned)ii < (unsigned)WX && (unsigned)jj < (unsigned)WY))
-                tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = 0.0f;
+                tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + aa] = 0.0f;
         }
     }
 
@@ -570,16 +672,16 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             load_qs(qS2);
             uint32_t negc2 = 0u;
             Roll q;
-            roll_row<WX>(col, y0, hasW, hasE, q.sW, q.sC, q.sE);
-            roll_row<WX>(col, y0 + 1, hasW, hasE, q.cW, q.cC, q.cE);
+            load_row(col + y0 * PX, q.sW, q.sC, q.sE);
+            load_row(col + (y0 + 1) * PX, q.cW, q.cC, q.cE);
 #pragma unroll
             for (int k = 0; k < RB; ++k) {
                 pns[k] = 0.0f;
-                if (y0 + k < WY) {
-                    roll_row<WX>(col, y0 + k + 2, hasW, hasE, q.nW, q.nC, q.nE);
-                    float So;
-                    const float pk = passive_rhs(q, qS2[k], gam, dwC, volC, repC, So, negc2);
-                    pns[k] = update(q, pk);
+                if (row_ok(k)) {
+                    load_row(col + (y0 + k + 2) * PX, q.nW, q.nC, q.nE);
+                    const float So = s_other_of(qS2[k], q.cC, negc2);
+                    const float pk = passive_head(q, gam, dwC, volC) - repC * q.cC * So;
+                    pns[k] = update(q.cC, 0.5f * (q.cE - q.cW), 0.5f * (q.nC - q.sC), pk);
                     q.sW = q.cW; q.sC = q.cC; q.sE = q.cE;
                     q.cW = q.nW; q.cC = q.nC; q.cE = q.nE;
                 }
@@ -590,28 +692,30 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         if (active) {
 #pragma unroll
             for (int k = 0; k < RB; ++k)
-                if (y0 + k < WY) phi_s[(y0 + k + 1) * WX + x] = pns[k];
+                if (row_ok(k)) phi_s[(y0 + k + 1) * PX + 4 + x] = pns[k];
         }
         __syncthreads();
         for (int i = tid; i < dwx * dwy; i += kNT) {
-            const int b = i / dwx, a = i - (i / dwx) * dwx;
-            const int ii = a + sx, jj = b + sy;
+            const int b = i / dwx, aa = i - (i / dwx) * dwx;
+            const int ii = aa + sx, jj = b + sy;
             const bool iin = (unsigned)ii < (unsigned)WX;
             const bool jin = (unsigned)jj < (unsigned)WY;
+            const float* pr = phi_s + (jj + 1) * PX + 4 + ii;
             float e = 0.0f, w = 0.0f, nn = 0.0f, ss = 0.0f;
             if (jin) {
-                if ((unsigned)(ii + 1) < (unsigned)WX) e = phi_s[(jj + 1) * WX + ii + 1];
-                if ((unsigned)(ii - 1) < (unsigned)WX) w = phi_s[(jj + 1) * WX + ii - 1];
+                if ((unsigned)(ii + 1) < (unsigned)WX) e = pr[1];
+                if ((unsigned)(ii - 1) < (unsigned)WX) w = pr[-1];
             }
             if (iin) {
-                if ((unsigned)(jj + 1) < (unsigned)WY) nn = phi_s[(jj + 2) * WX + ii];
-                if ((unsigned)(jj - 1) < (unsigned)WY) ss = phi_s[jj * WX + ii];
+                if ((unsigned)(jj + 1) < (unsigned)WY) nn = pr[PX];
+                if ((unsigned)(jj - 1) < (unsigned)WY) ss = pr[-PX];
             }
             const float pgx = 0.5f * (e - w), pgy = 0.5f * (nn - ss);
             aPer += (double)sqrtf(pgx * pgx + pgy * pgy);
         }
     }
-    reduce_commit<CLS>(A, n, sm, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
+    if (control) fetch4(A, sm, nf, nxt, step);
+    reduce_commit<CLS>(A, n, sm, c, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
                        pmaxb);
 }
 
@@ -621,14 +725,12 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
 // -------------------------------------------------------------------------
 template <int CLS>
 __device__ void cell_global(const pfh2d_step_args& A, int n,
-                            unsigned long long step, Smem& sm) {
+                            unsigned long long step, Smem& sm, Ctl& c, int nxt) {
     constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
     constexpr int TX0 = kShapes[CLS].tx0, TY0 = kShapes[CLS].ty0;
     const int tid = opaque_tid();
     const int L = A.L, P = A.P;
-    if (tid == 0) cell_scalars<CLS>(scalar_args(A), n, step, &sm.ctl);
-    __syncthreads();
-    const Ctl& c = sm.ctl;
+    if (tid >= kControlTid0) fetch_all(A, sm, nxt, step);
     const int gx0 = c.gx0, gy0 = c.gy0;
     const float* tin = A.phi_in + (size_t)n * kTileArea;
     float* tout = A.phi_out + (size_t)n * kTileArea;
@@ -655,7 +757,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         aIx += (double)(cc * gxv * So);
         aIy += (double)(cc * gyv * So);
     }
-    reduce_velocity(A, n, sm, aIx, aIy);
+    reduce_velocity(A, n, sm, c, aIx, aIy);
 
     const float gam = c.gam, dwC = c.dwC, volC = c.volC, repC = A.rep_coeff;
     const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
@@ -736,7 +838,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
     if (nonfin) flag_or(A.flags, PFH_FLAG_NONFINITE);
     if (clampf) flag_or(A.flags, PFH_FLAG_Q_CLAMP);
     if (ovf) flag_or(A.flags, PFH_FLAG_S_OVERFLOW);
-    reduce_commit<CLS>(A, n, sm, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
+    reduce_commit<CLS>(A, n, sm, c, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
                        pmaxb);
 }
 
@@ -745,9 +847,11 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
 // not raise the pressure of the hot class-0 path.
 template <int CLS>
 __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
-                                       unsigned long long step, Smem* sm) {
-    if constexpr (CLS <= kClassStagedMax) cell_staged<CLS>(*A, n, step, *sm);
-    else cell_global<CLS>(*A, n, step, *sm);
+                                       unsigned long long step, Smem* sm, int cur) {
+    if constexpr (CLS <= kClassStagedMax)
+        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
+    else
+        cell_global<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
 }
 
 // -------------------------------------------------------------------------
@@ -779,40 +883,29 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
     const unsigned long long step = *A.step_rd;
     if (tid == 0) sm.args = A;
 
+    // The control wavefront fetches each next cell one cell ahead (ticket,
+    // id, class, CellState, scalars) while the other wavefronts compute, so
+    // the queue's global round trips stay off the critical path.
+    if (tid >= kControlTid0) fetch_all(A, sm, 0, step);
+    __syncthreads();
+    int cur = 0;
     for (;;) {
-        // Thread 0 takes the ticket and also flags an unsupported class id
-        // here, so that every branch after the broadcast is workgroup-uniform
-        // with no single-thread work: a thread-0-only statement in the
-        // dispatch lets the control-flow structurizer fold it into the loop
-        // head, where the other lanes would re-read a stale ticket forever.
-        if (tid == 0) {
-            const unsigned long long k = atomicAdd(A.cursor_use, 1ull);
-            int nn = -1, cl = -1;
-            if (k < (unsigned long long)A.N) {
-                nn = (int)A.perm[k];
-                cl = (int)A.cell_cls[nn];
-                if (cl >= kNumClasses) {
-                    atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
-                    cl = kNumClasses;        // skipped below
-                }
-            }
-            sm.q_n = nn;
-            sm.q_cls = cl;
-        }
-        __syncthreads();
-        const int cls = __builtin_amdgcn_readfirstlane(sm.q_cls);
-        const int n = __builtin_amdgcn_readfirstlane(sm.q_n);
-        __syncthreads();                  // q_n/q_cls are rewritten next turn
+        const int cls = __builtin_amdgcn_readfirstlane(sm.slot[cur].cls);
+        const int n = __builtin_amdgcn_readfirstlane(sm.slot[cur].n);
         if (cls < 0) break;
         switch (cls) {
-            case 0: cell_staged<0>(A, n, step, sm); break;
-            case 1: cell_cold<1>(&sm.args, n, step, &sm); break;
-            case 2: cell_cold<2>(&sm.args, n, step, &sm); break;
-            case 3: cell_cold<3>(&sm.args, n, step, &sm); break;
-            case 4: cell_cold<4>(&sm.args, n, step, &sm); break;
-            case 5: cell_cold<5>(&sm.args, n, step, &sm); break;
-            default: break;                  // unsupported: flagged above
+            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1); break;
+            case 1: cell_cold<1>(&sm.args, n, step, &sm, cur); break;
+            case 2: cell_cold<2>(&sm.args, n, step, &sm, cur); break;
+            case 3: cell_cold<3>(&sm.args, n, step, &sm, cur); break;
+            case 4: cell_cold<4>(&sm.args, n, step, &sm, cur); break;
+            case 5: cell_cold<5>(&sm.args, n, step, &sm, cur); break;
+            default:                       // unsupported id, flagged at fetch
+                if (tid >= kControlTid0) fetch_all(A, sm, cur ^ 1, step);
+                __syncthreads();
+                break;
         }
+        cur ^= 1;
     }
 }
 
@@ -894,7 +987,7 @@ __global__ __launch_bounds__(kNT) void k_init_moments(pfh_cell_state* cell,
         }
     };
     if (cls <= kClassStagedMax) {
-        const int nb = kNT / wx, rb = (wy + nb - 1) / nb;
+        const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
         const int x = tid % wx, band = tid / wx;
         if (band < nb)
             for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);

@@ -27,6 +27,8 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kNT = 1024;
 constexpr int kWave = 64;
 constexpr int kWaves = kNT / kWave;
+// Threads that may own pixels in the LDS path (column bands).
+constexpr int kPixThreads = kNT;
 
 // ---- fixed tile and shape classes (include/params.cuh:121-147) ----
 constexpr int kTilePitch = 288;
@@ -72,12 +74,13 @@ template <int CLS> struct Staged {
     static constexpr int WY = kShapes[CLS].wy;
     static constexpr int TX0 = kShapes[CLS].tx0;
     static constexpr int TY0 = kShapes[CLS].ty0;
-    static constexpr int NB = kNT / WX;
+    static constexpr int NB = kPixThreads / WX;
     static constexpr int RB = (WY + NB - 1) / NB;
     static constexpr int NCH = WX / 4;                  // 16 B chunks per row
     static constexpr int NCHUNK = WY * NCH;
     static constexpr int KCH = (NCHUNK + kNT - 1) / kNT;
-    static constexpr int LDS_FLOATS = 4 + (WY + 2) * WX + 4;
+    static constexpr int PX = WX + 4;                   // LDS row pitch
+    static constexpr int LDS_FLOATS = (WY + 2) * PX + 4;
     static_assert(WX % 4 == 0 && TX0 % 4 == 0, "16 B phi rows");
     static_assert(NB * RB >= WY, "bands must cover the window");
 };
