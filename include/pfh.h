@@ -229,6 +229,180 @@ int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,
 int pfh2d_morton_sort(const pfh_cell_state* cell, uint32_t* perm, int N,
                       int M, int L, void* stream);
 
+/* ====================================================================== */
+/* ---- 3D: include/pf3d/kernels.cuh (periodic XYZ, cubic bricks) ---- */
+
+/* Sticky integrity flags: include/pf3d/kernels.cuh:55-91. CellState3D.flags
+ * holds them as bits; the global array holds one count per flag. */
+enum {
+    PFH3D_FLAG_S_OVERFLOW = 0,
+    PFH3D_FLAG_Q_CLAMP = 1,
+    PFH3D_FLAG_S_NEGATIVE = 2,
+    PFH3D_FLAG_NONFINITE = 3,
+    PFH3D_FLAG_V_NONPOSITIVE = 4,
+    PFH3D_FLAG_SUPPORT_EXHAUSTED = 5,
+    PFH3D_FLAG_SUPPORT_EDGE = 6,
+    PFH3D_FLAG_DESTINATION_CLIP = 7,
+    PFH3D_FLAG_INVALID_GEOMETRY = 8,
+    PFH3D_FLAG_SLAB_TOP_CONTACT = 9,
+    PFH3D_FLAG_COUNT = 10
+};
+#define PFH3D_FATAL_MASK 0x3BFu  /* all but SUPPORT_EDGE (kernels.cuh:74-83) */
+#define PFH3D_MAX_SHIFT 4        /* kMaximumShiftPerStep, src/pf3d/sim.cu:27 */
+#define PFH3D_MAX_EDGE 64        /* bricks up to 64^3 (one x group per lane) */
+
+/* Per-cell state: CellState3D, include/pf3d/kernels.cuh:98-137. 256 B. */
+typedef struct pfh3d_cell_state {
+    int64_t global_id;
+    int64_t origin_x, origin_y, origin_z;   /* unwrapped brick origin */
+    float polarity_x, polarity_y, polarity_z;
+    float velocity_x, velocity_y, velocity_z;
+    float gamma, v_A, R_tgt;
+    float phi_max;
+    uint32_t shift_ctr;
+    uint32_t tumble_ctr;
+    uint32_t flags;
+    uint32_t pad0;
+    int32_t bb_lo_x, bb_hi_x, bb_lo_y, bb_hi_y, bb_lo_z, bb_hi_z;
+    int32_t pending_shift_x, pending_shift_y, pending_shift_z;
+    uint32_t pad1;
+    double V, Cx, Cy, Cz, surface, Ix, Iy, Iz;
+    uint32_t storage_edge;                  /* must be 0: base brick */
+    uint32_t pad2;
+    double wall_overlap;
+    double outside_slit_volume;
+    uint32_t reserved[10];
+} __attribute__((aligned(64))) pfh3d_cell_state;
+
+/* One shard's moment reduction: MomentPartial3D (kernels.cuh:201-211). */
+typedef struct pfh3d_moment_partial {
+    double V, Cx, Cy, Cz, surface, Ix, Iy, Iz;
+    int32_t lo_x, hi_x, lo_y, hi_y, lo_z, hi_z;
+    uint32_t phi_max_bits, padding;
+} __attribute__((aligned(16))) pfh3d_moment_partial;
+
+/* Trajectory record: TrajPackedCell3D (kernels.cuh:318-328). */
+typedef struct pfh3d_traj_cell {
+    int64_t global_id;
+    double x, y, z;
+    double volume, surface, wall_overlap, outside_slit_volume;
+    float velocity_x, velocity_y, velocity_z;
+    float polarity_x, polarity_y, polarity_z;
+    float gamma, v_A, phi_max;
+} pfh3d_traj_cell;
+
+/* Arguments of one 3D step: the union of MeasureArgs3D and UpdateArgs3D
+ * (include/pf3d/kernels.cuh:232-316) for the periodic cube of side L with S
+ * row pitch P (= 32 ceil(L/32), src/pf3d/sim.cu:276). Fields and aggregates
+ * swap every begun step (src/pf3d/sim.cu:1632-1640). counters[0] is the
+ * accepted-step count (the Philox step, d_step_), counters[1] the begun-step
+ * count (the full-moment cadence, steps_done_). partials holds
+ * N*shards pfh3d_moment_partial and arrivals N uint32 words that are zero
+ * between launches (the library leaves them zero). */
+typedef struct pfh3d_step_args {
+    const float* phi_in;
+    float* phi_out;
+    const uint32_t* S_in;
+    uint32_t* S_out;
+    pfh3d_cell_state* cells;
+    pfh3d_cell_state* accepted;   /* accepted-state snapshot (d_accepted_cells_) */
+    uint32_t* flags;              /* PFH3D_FLAG_COUNT global counters */
+    uint64_t* counters;           /* [0] accepted steps, [1] begun steps */
+    pfh3d_moment_partial* partials;
+    uint32_t* arrivals;
+    int N, B, L, P;
+    int shards;                   /* measurement CTAs per cell, 1..B */
+    int max_shift;                /* 1..PFH3D_MAX_SHIFT */
+    int support_margin;
+    int full_moment_every;
+    float dt;
+    float bulk_scale;
+    float interaction_coeff;
+    float motility_coeff;
+    double V0;
+    double volume_scale;
+    double p_tumble;
+    uint64_t polarity_stream;
+    uint64_t aging_steps;
+} pfh3d_step_args;
+
+#ifndef __cplusplus
+_Static_assert(sizeof(pfh3d_cell_state) == 256, "CellState3D is 256 B");
+_Static_assert(offsetof(pfh3d_cell_state, storage_edge) == 192, "storage_edge moved");
+_Static_assert(sizeof(pfh3d_moment_partial) == 96, "MomentPartial3D is 96 B");
+_Static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
+#else
+static_assert(sizeof(pfh3d_cell_state) == 256, "CellState3D is 256 B");
+static_assert(offsetof(pfh3d_cell_state, storage_edge) == 192, "storage_edge moved");
+static_assert(sizeof(pfh3d_moment_partial) == 96, "MomentPartial3D is 96 B");
+static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
+#endif
+
+/* Replaces launch_initialize_spheres (kernels.cuh:417;
+ * src/pf3d/kernels.cu:1071-1122): origin = rint(centre - (B-1)/2) and
+ * phi = 0.5 (1 - tanh(k (r - seed_radius))), k = sqrt(7.5)/lambda; resets the
+ * per-cell dynamics fields. centres: 3*N doubles (x, y, z). phi_odd may be
+ * NULL; otherwise it is zeroed. */
+int pfh3d_init_spheres(float* phi_even, float* phi_odd, pfh3d_cell_state* cells,
+                       const double* centres, int N, int B, int L,
+                       float lambda, float seed_radius, void* stream);
+/* Replaces launch_clear_S (kernels.cuh:418-419). */
+int pfh3d_clear_S(uint32_t* S, int L, int P, void* stream);
+/* Replaces launch_scatter_current (kernels.cuh:420-421;
+ * src/pf3d/kernels.cu:1148-1186): S += q(phi) of every cell. */
+int pfh3d_scatter_current(const float* phi, pfh3d_cell_state* cells,
+                          uint32_t* S, uint32_t* flags, int N, int B, int L,
+                          int P, void* stream);
+/* Replaces launch_measure_cells / launch_measure_cells_only followed by
+ * launch_apply_cell_motion (kernels.cuh:436-447; src/pf3d/kernels.cu:
+ * 1591-1788): measures phi_in against S_in, commits V, centroid, support box,
+ * phi_max, interaction integral, surface (compute_surface) and the planned
+ * shift, then (apply_motion) the tumble and velocity. Reads counters[0] when
+ * apply_tumble. Uses phi_in, S_in, cells, flags, counters, partials, arrivals. */
+int pfh3d_measure_cells(const pfh3d_step_args* a, int apply_tumble,
+                        int compute_surface, int apply_motion, void* stream);
+/* Replaces launch_update_tiled_fast, launch_update_tiled_sharded_fast and
+ * launch_update_periodic_slab_walk_fast (kernels.cuh:506-533): the
+ * moment-free update phi_in -> phi_out with the pending recentring shift,
+ * scattering q(phi_out) into S_out (which the caller zeroed). Every
+ * destination voxel is written. slabs = z slabs per cell (0: automatic). */
+int pfh3d_update(const pfh3d_step_args* a, int slabs, void* stream);
+/* Replaces launch_repair_after_fatal, launch_finalize_origins,
+ * launch_restore_cells_after_fatal and launch_advance_step
+ * (kernels.cuh:470-481; src/pf3d/kernels.cu:2128-2263) as one launch. */
+int pfh3d_finish_step(const pfh3d_step_args* a, void* stream);
+/* Replaces Sim3D::begin_step + finish_step (src/pf3d/sim.cu:1460-1651) for
+ * the periodic base-brick path: snapshot, measurement (surface every
+ * full_moment_every begun steps), motion, S_out clear, update, repair /
+ * finalize / restore / advance. Three launches, no host synchronisation, no
+ * allocation: safe to capture into a HIP graph. */
+int pfh3d_step(const pfh3d_step_args* a, int slabs, void* stream);
+/* Replaces launch_pack_trajectory (kernels.cuh:482-485). */
+int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out,
+                          int N, void* stream);
+/* z slabs per cell that pfh3d_update uses when slabs == 0. */
+int pfh3d_update_slabs(int N, int B, int device);
+
+/* ---- 3D host-side set-up (no GPU work) ---- */
+/* automatic_brick_edge (include/pf3d/params.cuh:122-152). */
+int pfh3d_brick_edge(double radius, double lambda);
+/* domain_side_for (include/pf3d/params.cuh:224-233). */
+int pfh3d_domain_side(int num_cells, double radius, double rho);
+/* calibrate_seed_radius (include/pf3d/reference.hpp:712-745): bisection of
+ * the phi^2 volume of a centred sphere to 4 pi R^3 / 3 (relative tolerance
+ * 1e-10, at most 80 halvings). Returns the iteration count, or a negative
+ * status. */
+int pfh3d_calibrate_seed_radius(double radius, double lambda, int B,
+                                double* seed_radius);
+/* palmieri3d_sequential_centres (include/pf3d/palmieri_initializer.hpp:72-123):
+ * cell 0 at the box centre, then mt19937_64 proposals (x, y, z, high 53 bits)
+ * accepted at periodic distance >= radius, stored as binary32. Returns the
+ * number of candidates drawn, or a negative status. */
+int pfh3d_sequential_centres(int N, double side, double radius, uint64_t seed,
+                             float* x, float* y, float* z);
+/* initial_polarity (include/pf3d/rng.cuh:41-52), in double. */
+int pfh3d_initial_polarity(int64_t global_id, uint64_t stream, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@
 
 #include "../../include/pfh.h"
 #include "pfh_device.h"
+#include "pfh_host.h"
 
 namespace pfh {
 
@@ -1181,7 +1182,9 @@ __global__ __launch_bounds__(1024) void k_morton_sort(const pfh_cell_state* cell
 // -------------------------------------------------------------------------
 namespace {
 thread_local char g_last_error[256] = {0};
+}  // namespace
 
+namespace pfh_host {
 int hip_status(hipError_t e, const char* what) {
     if (e == hipSuccess) return PFH_OK;
     snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what,
@@ -1195,8 +1198,12 @@ int invalid(const char* what) {
     snprintf(g_last_error, sizeof(g_last_error), "invalid argument: %s", what);
     return PFH_ERR_INVALID;
 }
-hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
-}  // namespace
+int unsupported(const char* what) {
+    snprintf(g_last_error, sizeof(g_last_error), "%s", what);
+    return PFH_ERR_UNSUPPORTED;
+}
+}  // namespace pfh_host
+using namespace pfh_host;
 
 extern "C" {
 
@@ -1231,11 +1238,8 @@ int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
                       int observation) {
     (void)l2_base; (void)l2_bytes; (void)l2_hit_ratio;
     if (!a) return invalid("args");
-    if (observation != 0) {
-        snprintf(g_last_error, sizeof(g_last_error),
-                 "velocity observation is not part of this build");
-        return PFH_ERR_UNSUPPORTED;
-    }
+    if (observation != 0)
+        return unsupported("velocity observation is not part of this build");
     if (a->moments) return invalid("moments must be NULL");
     if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
         !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||

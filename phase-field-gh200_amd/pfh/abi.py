@@ -80,6 +80,65 @@ class KernelStats(C.Structure):
 
 assert C.sizeof(StepArgs) == 216, C.sizeof(StepArgs)
 
+# ---- 3D records (include/pf3d/kernels.cuh) ----
+FLAG3_COUNT = 10
+FLAG3_NAMES = ["S_overflow", "q_clamp", "S_negative", "nonfinite", "V_nonpositive",
+               "support_exhausted", "support_edge", "destination_clip",
+               "invalid_geometry", "slab_top_contact"]
+FATAL3_MASK = 0x3BF
+MAX_SHIFT_3D = 4
+MAX_EDGE_3D = 64
+
+# CellState3D (include/pf3d/kernels.cuh:98-137): 256 bytes.
+CELL3_DTYPE = np.dtype({
+    "names": ["global_id", "origin_x", "origin_y", "origin_z",
+              "polarity_x", "polarity_y", "polarity_z",
+              "velocity_x", "velocity_y", "velocity_z", "gamma", "v_A", "R_tgt",
+              "phi_max", "shift_ctr", "tumble_ctr", "flags", "pad0",
+              "bb_lo_x", "bb_hi_x", "bb_lo_y", "bb_hi_y", "bb_lo_z", "bb_hi_z",
+              "pending_shift_x", "pending_shift_y", "pending_shift_z", "pad1",
+              "V", "Cx", "Cy", "Cz", "surface", "Ix", "Iy", "Iz",
+              "storage_edge", "pad2", "wall_overlap", "outside_slit_volume", "reserved"],
+    "formats": ["<i8", "<i8", "<i8", "<i8"] + ["<f4"] * 10 + ["<u4"] * 4 + ["<i4"] * 9
+               + ["<u4"] + ["<f8"] * 8 + ["<u4", "<u4", "<f8", "<f8", ("<u4", (10,))],
+    "offsets": [0, 8, 16, 24, 32, 36, 40, 44, 48, 52, 56, 60, 64, 68, 72, 76, 80, 84,
+                88, 92, 96, 100, 104, 108, 112, 116, 120, 124, 128, 136, 144, 152, 160,
+                168, 176, 184, 192, 196, 200, 208, 216],
+    "itemsize": 256,
+})
+
+# TrajPackedCell3D (include/pf3d/kernels.cuh:318-328): 104 bytes.
+TRAJ3_DTYPE = np.dtype([("global_id", "<i8"), ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+                        ("volume", "<f8"), ("surface", "<f8"), ("wall_overlap", "<f8"),
+                        ("outside_slit_volume", "<f8"),
+                        ("velocity_x", "<f4"), ("velocity_y", "<f4"), ("velocity_z", "<f4"),
+                        ("polarity_x", "<f4"), ("polarity_y", "<f4"), ("polarity_z", "<f4"),
+                        ("gamma", "<f4"), ("v_A", "<f4"), ("phi_max", "<f4"),
+                        ("_pad", "<u4")])
+PARTIAL3_BYTES = 96
+
+
+class StepArgs3(C.Structure):
+    """pfh3d_step_args (MeasureArgs3D + UpdateArgs3D, include/pf3d/kernels.cuh:232-316)."""
+    _fields_ = [
+        ("phi_in", C.c_void_p), ("phi_out", C.c_void_p),
+        ("S_in", C.c_void_p), ("S_out", C.c_void_p),
+        ("cells", C.c_void_p), ("accepted", C.c_void_p),
+        ("flags", C.c_void_p), ("counters", C.c_void_p),
+        ("partials", C.c_void_p), ("arrivals", C.c_void_p),
+        ("N", C.c_int), ("B", C.c_int), ("L", C.c_int), ("P", C.c_int),
+        ("shards", C.c_int), ("max_shift", C.c_int), ("support_margin", C.c_int),
+        ("full_moment_every", C.c_int),
+        ("dt", C.c_float), ("bulk_scale", C.c_float), ("interaction_coeff", C.c_float),
+        ("motility_coeff", C.c_float),
+        ("V0", C.c_double), ("volume_scale", C.c_double), ("p_tumble", C.c_double),
+        ("polarity_stream", C.c_uint64), ("aging_steps", C.c_uint64),
+    ]
+
+
+assert C.sizeof(StepArgs3) == 168, C.sizeof(StepArgs3)
+assert TRAJ3_DTYPE.itemsize == 104
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -103,6 +162,20 @@ SIGNATURES = {
     "pfh2d_verify_S": (_I, [_P, _SZ, _P, _P]),
     "pfh2d_pack_traj": (_I, [_P, _P, _P, _I, _I, _P]),
     "pfh2d_morton_sort": (_I, [_P, _P, _I, _I, _I, _P]),
+    "pfh3d_init_spheres": (_I, [_P, _P, _P, _P, _I, _I, _I, _F, _F, _P]),
+    "pfh3d_clear_S": (_I, [_P, _I, _I, _P]),
+    "pfh3d_scatter_current": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    "pfh3d_measure_cells": (_I, [C.POINTER(StepArgs3), _I, _I, _I, _P]),
+    "pfh3d_update": (_I, [C.POINTER(StepArgs3), _I, _P]),
+    "pfh3d_finish_step": (_I, [C.POINTER(StepArgs3), _P]),
+    "pfh3d_step": (_I, [C.POINTER(StepArgs3), _I, _P]),
+    "pfh3d_pack_trajectory": (_I, [_P, _P, _I, _P]),
+    "pfh3d_update_slabs": (_I, [_I, _I, _I]),
+    "pfh3d_brick_edge": (_I, [C.c_double, C.c_double]),
+    "pfh3d_domain_side": (_I, [_I, C.c_double, C.c_double]),
+    "pfh3d_calibrate_seed_radius": (_I, [C.c_double, C.c_double, _I, C.POINTER(C.c_double)]),
+    "pfh3d_sequential_centres": (_I, [_I, C.c_double, C.c_double, C.c_uint64, _P, _P, _P]),
+    "pfh3d_initial_polarity": (_I, [C.c_int64, C.c_uint64, _P]),
 }
 
 # Entry points the ABI header declares; tests check the library exports all.
