@@ -13,6 +13,11 @@ kernel from its algorithmic bytes (per cell-step 12*W + 8*Q + 4*P*L/N: read
 the phi window, read the S window, write the phi window, atomic
 read-modify-write of S where q != 0, and the amortized clear of one S slot),
 and the CPU oracle's throughput on a bounded sample of the same workload.
+
+The "pf3d" object beside it measures the 3D periodic step on 256^3 with 128
+cells (R=12, lambda=3.5, brick 56) the same way, with algorithmic bytes per
+cell-step 16*B^3 + 8*Q3 + 4*P*L^2/N (two phi reads, one phi write and one S
+read over the brick, the Q5.27 atomics, the amortized S clear).
 """
 from __future__ import annotations
 
@@ -85,6 +90,89 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
                       f"after 2 warm-up steps ({el:.1f} s wall)"}
 
 
+def q_nonzero_3d(sim) -> int:
+    """Voxels with q(phi) != 0 over every brick (GPU count)."""
+    import torch
+    phi = sim.phi[sim.steps_done % 2]
+    q = torch.round(torch.clamp(phi * phi, max=4.0) * 134217728.0)
+    return int((q != 0).sum().item())
+
+
+def cpu_baseline_3d(wl, shards: int, budget_s: float = 10.0) -> dict:
+    """The 3D CPU oracle (binary32, the kernels' operation order) on the same
+    workload, timed on this host's cores after one warm-up step."""
+    from oracle import pfo3
+    threads = min(16, os.cpu_count() or 1)
+    p = wl.params
+    orc = pfo3.OracleSim3D(p, wl.cx, wl.cy, wl.cz, shards=shards, threads=threads)
+    orc.step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        orc.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 100:
+            break
+    return {"value": p.num_cells * steps / el, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps x {p.num_cells} cells of the same workload after "
+                      f"1 warm-up step ({el:.1f} s wall)"}
+
+
+def bench_3d(steps: int, warmup: int, with_cpu: bool) -> dict:
+    """Throughput of the 3D periodic step (measure + update + finish launches,
+    replayed from two-step HIP graphs) on the 256^3 / 128-cell workload."""
+    import torch
+    from pfh import params3d, sim3d
+    wl = params3d.workload_c3()
+    sim = sim3d.Sim3D(wl, device="cuda:0")
+    sim.capture_graph()
+    warm = max(2, (warmup // 2) * 2)
+    steps = max(2, (steps // 2) * 2)
+    sim.run(warm)
+    sim.sync()
+    q0 = q_nonzero_3d(sim)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(sim.stream)
+    sim.run(steps)
+    e1.record(sim.stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    q1 = q_nonzero_3d(sim)
+    if sim.fatal():
+        return {"error": "fatal integrity flags", "flags": sim.flag_counts().tolist()}
+    N, B, L, P = sim.N, sim.B, sim.L, sim.P
+    ms_per_step = ms / steps
+    Qbar = 0.5 * (q0 + q1)
+    bytes_per_step = 16.0 * N * B ** 3 + 8.0 * Qbar + 4.0 * P * L * L
+    achieved = bytes_per_step / (ms_per_step * 1e-3) / 1e9
+    out = {
+        "metric": "cell-steps/s, 3D periodic 128 cells",
+        "value": N * steps / (ms / 1e3),
+        "unit": "cell-steps/s",
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": ms_per_step,
+        "dtype": "f32",
+        "config": {"workload": wl.name, "lattice": [L, L, L], "cells": N,
+                   "radius": wl.params.target_radius, "lambda": wl.params.lambda_,
+                   "brick_edge": B, "automatic_brick_edge": wl.params.automatic_brick_edge(),
+                   "placement": "sequential, min separation R, seed 1234",
+                   "measure_shards": sim.shards, "update_slabs": sim.slabs,
+                   "graph_steps": sim3d.GRAPH_BODY},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_step": bytes_per_step,
+                     "q_nonzero_per_step": Qbar},
+    }
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_3d(wl, sim.shards)
+    return out
+
+
 def calibrate_copy(mib: int) -> int:
     import ctypes as C
 
@@ -109,6 +197,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1200)
     ap.add_argument("--warmup", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["all", "2d", "3d"], default="all",
+                    help="2d: the headline 2048^2 step only; 3d: the 256^3 step only")
+    ap.add_argument("--steps3d", type=int, default=600)
+    ap.add_argument("--warmup3d", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 8 plain copies of MIB MiB "
                          "(the known-size kernel that scales hardware byte counters)")
@@ -120,6 +212,11 @@ def main() -> int:
               "(the per-cell step does not shard across GPUs in this build)",
               file=sys.stderr)
         return 2
+
+    if args.workload == "3d":
+        print(json.dumps(bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)),
+              flush=True)
+        return 0
 
     import torch
     from pfh import params, sim2d
@@ -181,6 +278,10 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl)
+    del sim
+    torch.cuda.empty_cache()
+    if args.workload == "all":
+        out["pf3d"] = bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
     return 0
 
