@@ -403,6 +403,66 @@ int pfh3d_sequential_centres(int N, double side, double radius, uint64_t seed,
 /* initial_polarity (include/pf3d/rng.cuh:41-52), in double. */
 int pfh3d_initial_polarity(int64_t global_id, uint64_t stream, double out[3]);
 
+/* ====================================================================== */
+/* ---- learned surrogate of the 2D step ----
+ * No counterpart in the reference: a BF16 CNN (4 -> 32 -> 32 -> 32 -> 1
+ * channels, 3x3 convolutions, ReLU, residual output) that advances a cell's
+ * 96x96 crop by K plain steps, with its training kernels and the
+ * accept/fallback kernels. Activations are NHWC bfloat16 (uint16_t bit
+ * patterns): the input has 8 channels (phi, S_other, cos theta, sin theta,
+ * four zero channels), hidden layers 32. MFMA-layer weights are [32][9][cin]
+ * (tap = 3 ky + kx), the output layer's [9][32] fp32. Crops are centred on
+ * the rounded centroid of the cell's stored field (include/kernels.cuh:27-57
+ * CellState, tile layout include/params.cuh:121-147). */
+#define PFH_SUR_CROP 96
+
+/* Network input X [N][96][96][8], exact phi crop phic [N][96][96] and the
+ * crop origins origin [N][2] (world pixel of crop (0,0)). S_other =
+ * (S - q(phi)) 2^-27 is exact. */
+int pfh_sur_crop(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                 const uint32_t* S, int N, int L, int P, uint16_t* X, float* phic,
+                 int* origin, void* stream);
+/* phi of each cell on the recorded crop windows (after the cells moved). */
+int pfh_sur_recrop(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                   const int* origin, int N, int L, float* out, void* stream);
+/* Y = relu?(conv3x3(X, W) + bias) [* (mask > 0)] on MFMA; cin = 8 or 32,
+ * bias and mask may be NULL. */
+int pfh_sur_conv(int cin, const uint16_t* X, const uint16_t* W, const float* bias,
+                 const uint16_t* mask, uint16_t* Y, int ncrops, int relu, void* stream);
+/* Output layer: dphi = b4 + conv3x3(H, W4); out = phic + dphi (either output
+ * may be NULL). */
+int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic, float* dphi,
+                float* out, int ncrops, void* stream);
+/* Squared-error loss per crop (fp64) and g = scale (dphi - target). */
+int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
+                 int ncrops, float scale, void* stream);
+/* Output-layer backward: dH (bf16, masked by H > 0) and per-crop partials of
+ * dW4 (288) then db4 (1). */
+int pfh_sur_out_bwd(const float* g, const uint16_t* H, const float* W4, uint16_t* dH,
+                    float* dW4_part, int ncrops, void* stream);
+/* Per-crop partials of dW [32][9][cin] then db [32] of an MFMA layer. */
+int pfh_sur_wgrad(int cin, const uint16_t* dY, const uint16_t* X, float* part, int ncrops,
+                  void* stream);
+/* out[e] = sum over crops n (ascending) of part[n][e]. */
+int pfh_sur_reduce(const float* part, int ncrops, int width, float* out, void* stream);
+/* Backward-data weights of a 32 -> 32 layer: Wt[ci][t][co] = W[co][8-t][ci]. */
+int pfh_sur_flip(const uint16_t* W, uint16_t* Wt, void* stream);
+/* Adam step on fp32 master weights; wb (may be NULL) receives bf16 copies.
+ * c1, c2: bias corrections 1 - beta^t. */
+int pfh_sur_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int n, float lr,
+                 float b1, float b2, float eps, float c1, float c2, void* stream);
+/* Per-crop relative L2 error of a against b. */
+int pfh_sur_rel_l2(const float* a, const float* b, float* out, int ncrops, void* stream);
+/* Write the predicted crops of accepted cells into their tiles' active
+ * windows (window pixels outside the crop become 0). */
+int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                      const int* origin, const float* pred, const int* accept, int N, int L,
+                      void* stream);
+/* The polarity draws of K skipped plain steps, steps step0 .. step0+K-1
+ * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. */
+int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+                   unsigned long long polarity_seed, double p_tumble, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

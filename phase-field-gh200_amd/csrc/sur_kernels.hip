@@ -1,0 +1,636 @@
+// Learned surrogate of the 2D step on gfx950: per-cell crops, a 4-layer 3x3
+// CNN (4 -> 32 -> 32 -> 32 -> 1 channels, residual output) in BF16 with FP32
+// accumulation, its training kernels, and the accept/fallback kernels.
+//
+// The reference has no learned component; the crop builds on its tile layout
+// (include/params.cuh:121-147) and CellState (include/kernels.cuh:27-57), and
+// the accepted fields are re-measured with the plain path's own kernels
+// (k_scatter_all / k_init_moments semantics, src/kernels.cu:1098-1190).
+//
+// Layout: activations NHWC bf16, one 96x96 crop per cell, channels fastest
+// (8 for the input: phi, S_other, cos theta, sin theta, 4 zero channels; 32
+// for hidden layers). Weights [out][tap][in] bf16 for the MFMA layers, tap =
+// 3 ky + kx.
+//
+// k_sur_conv: implicit GEMM on MFMA (v_mfma_f32_32x32x16_bf16 for 32 input
+// channels, v_mfma_f32_32x32x8_bf16 for 8): M = 32 pixels of a row, N = 32
+// output channels, K = 9 taps x input channels. Persistent workgroups keep
+// the weights in LDS and walk (crop, 4-row block) items; each item stages a
+// 6 x 98 pixel halo tile in LDS, each wave owns one output row (three
+// 32-pixel tiles sharing every B fragment). The same kernel computes the
+// backward-data pass with tap-flipped, transposed weights and a ReLU mask.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/pfh.h"
+#include "pfh_device.h"
+#include "pfh_host.h"
+
+namespace sur {
+
+constexpr int kCrop = 96;
+constexpr int kCropPix = kCrop * kCrop;
+constexpr int kC0 = 8;     // input channels (4 used)
+constexpr int kCH = 32;    // hidden channels
+constexpr int kRowsPerItem = 4;
+constexpr int kItemsPerCrop = kCrop / kRowsPerItem;   // 24
+constexpr int kHaloCols = kCrop + 2;                  // 98
+
+typedef uint16_t bf16_t;
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// binary32 -> bfloat16, round to nearest even (NaN stays NaN)
+__host__ __device__ inline bf16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (bf16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h) {
+    return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// ---------------------------------------------------------------------------
+// crops
+// ---------------------------------------------------------------------------
+// Crop origin: the cell's rounded centroid minus 48, wrapped into [0, L).
+__device__ inline void crop_origin(const pfh_cell_state& c, int L, int* ox, int* oy) {
+    double cx = (double)c.gx0, cy = (double)c.gy0;
+    if (c.V > 0.0) {
+        cx += c.Cx / c.V;
+        cy += c.Cy / c.V;
+    }
+    *ox = pfh::wrapi((int)rint(cx) - kCrop / 2, L);
+    *oy = pfh::wrapi((int)rint(cy) - kCrop / 2, L);
+}
+
+// phi of cell n at world pixel (wx, wy): the tile's active window, else 0
+__device__ inline float cell_phi(const float* tile, const pfh_cell_state& c, int cls, int L,
+                                 int wx, int wy) {
+    const pfh::Shape s = pfh::shape_of(cls);
+    int a = wx - c.gx0;
+    if (a < 0) a += L;
+    int b = wy - c.gy0;
+    if (b < 0) b += L;
+    if (a >= s.wx || b >= s.wy) return 0.0f;
+    return tile[(size_t)(s.ty0 + b) * pfh::kTilePitch + s.tx0 + a];
+}
+
+// One workgroup per cell: network input (bf16 NHWC, 8 channels), the exact
+// phi crop (fp32) and the crop origin.
+__global__ __launch_bounds__(256) void k_sur_crop(const float* phi, const pfh_cell_state* cells,
+                                                  const uint8_t* cls, const uint32_t* S,
+                                                  int L, int P, bf16_t* X, float* phic,
+                                                  int* origin) {
+    const int n = (int)blockIdx.x;
+    const pfh_cell_state c = cells[n];
+    int ox, oy;
+    crop_origin(c, L, &ox, &oy);
+    if (threadIdx.x == 0) {
+        origin[2 * n] = ox;
+        origin[2 * n + 1] = oy;
+    }
+    double sn, cs;
+    pfh::sincos_d((double)c.theta, &sn, &cs);
+    const bf16_t bc = f2bf((float)cs), bs = f2bf((float)sn);
+    const float* tile = phi + (size_t)n * pfh::kTileArea;
+    const int k = (int)cls[n];
+    for (int p = (int)threadIdx.x; p < kCropPix; p += (int)blockDim.x) {
+        const int i = p % kCrop, j = p / kCrop;
+        int wx = ox + i;
+        if (wx >= L) wx -= L;
+        int wy = oy + j;
+        if (wy >= L) wy -= L;
+        const float v = cell_phi(tile, c, k, L, wx, wy);
+        const uint32_t q = pfh::q_of(v);
+        const uint32_t s = S[(size_t)wy * P + wx];
+        const float other = s >= q ? (float)(s - q) * pfh::kQInv : 0.0f;
+        bf16_t* o = X + ((size_t)n * kCropPix + p) * kC0;
+        u32x4 w;
+        w[0] = (uint32_t)f2bf(v) | ((uint32_t)f2bf(other) << 16);
+        w[1] = (uint32_t)bc | ((uint32_t)bs << 16);
+        w[2] = 0u;
+        w[3] = 0u;
+        *reinterpret_cast<u32x4*>(o) = w;
+        phic[(size_t)n * kCropPix + p] = v;
+    }
+}
+
+// The same window of the field after the surrogate interval: phi crop at the
+// recorded origins (the cell's window may have moved meanwhile).
+__global__ __launch_bounds__(256) void k_sur_recrop(const float* phi, const pfh_cell_state* cells,
+                                                    const uint8_t* cls, const int* origin, int L,
+                                                    float* out) {
+    const int n = (int)blockIdx.x;
+    const pfh_cell_state c = cells[n];
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    const float* tile = phi + (size_t)n * pfh::kTileArea;
+    const int k = (int)cls[n];
+    for (int p = (int)threadIdx.x; p < kCropPix; p += (int)blockDim.x) {
+        const int i = p % kCrop, j = p / kCrop;
+        int wx = ox + i;
+        if (wx >= L) wx -= L;
+        int wy = oy + j;
+        if (wy >= L) wy -= L;
+        out[(size_t)n * kCropPix + p] = cell_phi(tile, c, k, L, wx, wy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA convolution
+// ---------------------------------------------------------------------------
+template <int CIN>
+struct ConvShape {
+    static constexpr int K = 9 * CIN;
+    static constexpr int KS = CIN == 8 ? 8 : 16;      // k per MFMA
+    static constexpr int NKS = K / KS;
+    static constexpr int PIX = CIN + 8;               // LDS pixel pitch (bf16)
+    static constexpr int WROW = K + 8;                // LDS weight row pitch (bf16)
+    static constexpr int HALO = (kRowsPerItem + 2) * kHaloCols;
+};
+
+template <int CIN>
+__device__ __forceinline__ f32x16 mfma_step(const bf16_t* a, const bf16_t* b, f32x16 acc) {
+    if constexpr (CIN == 8) {
+        const s16x4 av = *reinterpret_cast<const s16x4*>(a);
+        const s16x4 bv = *reinterpret_cast<const s16x4*>(b);
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    } else {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(a);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(b);
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+}
+
+// Y = act(conv3x3(X, W) + bias) [* (mask > 0)], zero padding at the crop edge.
+// X: [ncrops][96][96][CIN] bf16; W: [32][9*CIN] bf16; Y: [ncrops][96][96][32].
+template <int CIN>
+__global__ __launch_bounds__(256) void k_sur_conv(const bf16_t* X, const bf16_t* W,
+                                                  const float* bias, const bf16_t* mask,
+                                                  bf16_t* Y, int ncrops, int relu) {
+    using SH = ConvShape<CIN>;
+    __shared__ __align__(16) bf16_t wl[kCH * SH::WROW];
+    __shared__ __align__(16) bf16_t xl[SH::HALO * SH::PIX];
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, col = lane & 31;
+
+    // weights -> LDS once (16 B chunks)
+    for (int i = tid; i < kCH * (SH::K / 8); i += 256) {
+        const int co = i / (SH::K / 8), k8 = i - co * (SH::K / 8);
+        *reinterpret_cast<u32x4*>(wl + co * SH::WROW + 8 * k8) =
+            *reinterpret_cast<const u32x4*>(W + (size_t)co * SH::K + 8 * k8);
+    }
+    float bco = bias ? bias[col] : 0.0f;
+
+    const int items = ncrops * kItemsPerCrop;
+    for (int item = (int)blockIdx.x; item < items; item += (int)gridDim.x) {
+        const int crop = item / kItemsPerCrop;
+        const int y0 = (item - crop * kItemsPerCrop) * kRowsPerItem;
+        __syncthreads();
+        // stage rows y0-1 .. y0+4, columns -1 .. 96 (8-channel chunks)
+        constexpr int CH8 = CIN / 8;
+        for (int i = tid; i < SH::HALO * CH8; i += 256) {
+            const int px = i / CH8, c8 = i - px * CH8;
+            const int r = px / kHaloCols, cc = px - r * kHaloCols;
+            const int y = y0 - 1 + r, x = cc - 1;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if ((unsigned)y < (unsigned)kCrop && (unsigned)x < (unsigned)kCrop)
+                v = *reinterpret_cast<const u32x4*>(
+                    X + (((size_t)crop * kCrop + y) * kCrop + x) * CIN + 8 * c8);
+            *reinterpret_cast<u32x4*>(xl + px * SH::PIX + 8 * c8) = v;
+        }
+        __syncthreads();
+
+        f32x16 acc0 = {}, acc1 = {}, acc2 = {};
+#pragma unroll 3
+        for (int s = 0; s < SH::NKS; ++s) {
+            const int k0 = s * SH::KS;
+            const int t = k0 / CIN, c0 = k0 - t * CIN;
+            const int ky = t / 3, kx = t - 3 * ky;
+            const bf16_t* bp = wl + col * SH::WROW + k0 + (SH::KS / 2) * half;
+            const bf16_t* ap = xl + ((wave + ky) * kHaloCols + col + kx) * SH::PIX + c0 +
+                               (SH::KS / 2) * half;
+            acc0 = mfma_step<CIN>(ap, bp, acc0);
+            acc1 = mfma_step<CIN>(ap + 32 * SH::PIX, bp, acc1);
+            acc2 = mfma_step<CIN>(ap + 64 * SH::PIX, bp, acc2);
+        }
+
+        // epilogue: lane holds output channel `col`, pixel rows 8j + 4 half + i
+        const int y = y0 + wave;
+        const size_t rowbase = ((size_t)crop * kCrop + y) * kCrop;
+        for (int tile = 0; tile < 3; ++tile) {
+            const f32x16 a = tile == 0 ? acc0 : tile == 1 ? acc1 : acc2;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int px = 32 * tile + 8 * (r >> 2) + 4 * half + (r & 3);
+                const size_t o = (rowbase + px) * kCH + col;
+                float v = a[r] + bco;
+                if (mask && !(bf2f(mask[o]) > 0.0f)) v = 0.0f;
+                if (relu) v = fmaxf(v, 0.0f);
+                Y[o] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// output layer (32 -> 1), loss and their gradients (VALU, fp32)
+// ---------------------------------------------------------------------------
+// dphi = b4 + sum_{tap, ci} W4[tap][ci] h3[p + off(tap)][ci]; out = phic + dphi
+__global__ __launch_bounds__(256) void k_sur_out(const bf16_t* H, const float* W4, float b4,
+                                                 const float* phic, float* dphi, float* out,
+                                                 int ncrops) {
+    __shared__ float w[9 * kCH];
+    for (int i = (int)threadIdx.x; i < 9 * kCH; i += 256) w[i] = W4[i];
+    __syncthreads();
+    const size_t total = (size_t)ncrops * kCropPix;
+    for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total;
+         g += (size_t)gridDim.x * 256) {
+        const int crop = (int)(g / kCropPix), p = (int)(g - (size_t)crop * kCropPix);
+        const int x = p % kCrop, y = p / kCrop;
+        float acc = 0.0f;
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+            if ((unsigned)yy >= (unsigned)kCrop || (unsigned)xx >= (unsigned)kCrop) continue;
+            const bf16_t* h = H + (((size_t)crop * kCrop + yy) * kCrop + xx) * kCH;
+            for (int c8 = 0; c8 < 4; ++c8) {
+                const u32x4 v = *reinterpret_cast<const u32x4*>(h + 8 * c8);
+                for (int k = 0; k < 4; ++k) {
+                    acc += w[t * kCH + 8 * c8 + 2 * k] * bf2f((bf16_t)(v[k] & 0xFFFFu));
+                    acc += w[t * kCH + 8 * c8 + 2 * k + 1] * bf2f((bf16_t)(v[k] >> 16));
+                }
+            }
+        }
+        const float d = acc + b4;
+        if (dphi) dphi[g] = d;
+        if (out) out[g] = phic[g] + d;
+    }
+}
+
+// Per-crop loss partial and g = 2 (dphi - target) / count
+__global__ __launch_bounds__(256) void k_sur_loss(const float* dphi, const float* target,
+                                                  float* g, double* crop_loss, float scale) {
+    __shared__ double red[4];
+    const int n = (int)blockIdx.x;
+    double acc = 0.0;
+    for (int p = (int)threadIdx.x; p < kCropPix; p += 256) {
+        const size_t i = (size_t)n * kCropPix + p;
+        const float e = dphi[i] - target[i];
+        g[i] = scale * e;
+        acc += (double)e * (double)e;
+    }
+    acc = pfh::wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) crop_loss[n] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Backward of the output layer: dH3 (masked by h3 > 0, bf16), and per-crop
+// partials of dW4 [9][32] followed by db4.
+__global__ __launch_bounds__(256) void k_sur_out_bwd(const float* g, const bf16_t* H,
+                                                     const float* W4, bf16_t* dH,
+                                                     float* dW4_part) {
+    __shared__ float w[9 * kCH];
+    const int n = (int)blockIdx.x, tid = (int)threadIdx.x;
+    for (int i = tid; i < 9 * kCH; i += 256) w[i] = W4[i];
+    __syncthreads();
+    const float* gn = g + (size_t)n * kCropPix;
+    // dH[q][ci] = sum_t g[q - off(t)] W4[t][ci]: one thread per (q, 8 channels)
+    for (int i = tid; i < kCropPix * 4; i += 256) {
+        const int q = i >> 2, c8 = i & 3;
+        const int x = q % kCrop, y = q / kCrop;
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y - (t / 3 - 1), xx = x - (t % 3 - 1);
+            if ((unsigned)yy >= (unsigned)kCrop || (unsigned)xx >= (unsigned)kCrop) continue;
+            const float gv = gn[yy * kCrop + xx];
+            for (int k = 0; k < 8; ++k) acc[k] += gv * w[t * kCH + 8 * c8 + k];
+        }
+        const size_t o = ((size_t)n * kCropPix + q) * kCH + 8 * c8;
+        for (int k = 0; k < 8; ++k) dH[o + k] = f2bf(bf2f(H[o + k]) > 0.0f ? acc[k] : 0.0f);
+    }
+    // dW4[t][ci] = sum_p g[p] h3[p + off(t)][ci] (entries 0..287), db4 (entry 288)
+    for (int e = tid; e < 9 * kCH + 1; e += 256) {
+        const int tt = e / kCH, cc = e - tt * kCH;
+        float a = 0.0f;
+        for (int y = 0; y < kCrop; ++y)
+            for (int x = 0; x < kCrop; ++x) {
+                const float gv = gn[y * kCrop + x];
+                if (tt == 9) {
+                    a += gv;
+                    continue;
+                }
+                const int yy = y + tt / 3 - 1, xx = x + tt % 3 - 1;
+                if ((unsigned)yy >= (unsigned)kCrop || (unsigned)xx >= (unsigned)kCrop) continue;
+                a += gv * bf2f(H[(((size_t)n * kCrop + yy) * kCrop + xx) * kCH + cc]);
+            }
+        dW4_part[(size_t)n * (9 * kCH + 1) + e] = a;
+    }
+}
+
+// Per-crop partials of dW[co][tap][ci] and db[co] for an MFMA layer:
+// dW = sum_p dY[p][co] X[p + off(tap)][ci], db = sum_p dY[p][co].
+template <int CIN>
+__global__ __launch_bounds__(256) void k_sur_wgrad(const bf16_t* dY, const bf16_t* X,
+                                                   float* part) {
+    constexpr int NW = kCH * 9 * CIN;
+    const int n = (int)blockIdx.x, tid = (int)threadIdx.x;
+    __shared__ float dyl[kCrop * kCH];            // one row of dY (fp32)
+    __shared__ float xl[3 * kHaloCols * CIN];     // three rows of X with a zero halo
+    constexpr int PER = (NW + kCH + 255) / 256;
+    float acc[PER];
+    for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
+    for (int y = 0; y < kCrop; ++y) {
+        __syncthreads();
+        for (int i = tid; i < kCrop * kCH; i += 256)
+            dyl[i] = bf2f(dY[((size_t)n * kCrop + y) * kCrop * kCH + i]);
+        for (int i = tid; i < 3 * kHaloCols * CIN; i += 256) {
+            const int r = i / (kHaloCols * CIN), rem = i - r * kHaloCols * CIN;
+            const int cc = rem / CIN, ci = rem - cc * CIN;
+            const int yy = y + r - 1, xx = cc - 1;
+            xl[i] = ((unsigned)yy < (unsigned)kCrop && (unsigned)xx < (unsigned)kCrop)
+                        ? bf2f(X[(((size_t)n * kCrop + yy) * kCrop + xx) * CIN + ci]) : 0.0f;
+        }
+        __syncthreads();
+        for (int k = 0; k < PER; ++k) {
+            const int e = tid + 256 * k;
+            if (e >= NW + kCH) break;
+            float a = acc[k];
+            if (e < NW) {
+                const int co = e / (9 * CIN), rem = e - co * 9 * CIN;
+                const int t = rem / CIN, ci = rem - t * CIN;
+                const int ky = t / 3, kx = t - 3 * ky;
+                const float* xr = xl + (ky * kHaloCols + kx) * CIN + ci;
+                for (int x = 0; x < kCrop; ++x) a += dyl[x * kCH + co] * xr[x * CIN];
+            } else {
+                const int co = e - NW;
+                for (int x = 0; x < kCrop; ++x) a += dyl[x * kCH + co];
+            }
+            acc[k] = a;
+        }
+    }
+    for (int k = 0; k < PER; ++k) {
+        const int e = tid + 256 * k;
+        if (e < NW + kCH) part[(size_t)n * (NW + kCH) + e] = acc[k];
+    }
+}
+
+// Fixed-order sum of per-crop partials: out[e] = sum_n part[n][e] (ascending n)
+__global__ void k_sur_reduce(const float* part, int ncrops, int width, float* out) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= width) return;
+    float a = 0.0f;
+    for (int n = 0; n < ncrops; ++n) a += part[(size_t)n * width + e];
+    out[e] = a;
+}
+
+// W'[ci][tap][co] = W[co][8 - tap][ci] (32 x 32 layers): weights of the
+// backward-data convolution
+__global__ void k_sur_flip(const bf16_t* W, bf16_t* Wt) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= kCH * 9 * kCH) return;
+    const int ci = e / (9 * kCH), rem = e - ci * 9 * kCH;
+    const int t = rem / kCH, co = rem - t * kCH;
+    Wt[e] = W[((size_t)co * 9 + (8 - t)) * kCH + ci];
+}
+
+// Adam on fp32 master weights, bf16 copy for compute (wb may be null)
+__global__ void k_sur_adam(float* w, const float* g, float* m, float* v, bf16_t* wb, int n,
+                           float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+// ---------------------------------------------------------------------------
+// acceptance and write-back
+// ---------------------------------------------------------------------------
+// Per-crop relative L2 of a against b: sqrt(sum (a-b)^2 / sum b^2)
+__global__ __launch_bounds__(256) void k_sur_rel_l2(const float* a, const float* b,
+                                                    float* out) {
+    __shared__ double r1[4], r2[4];
+    const int n = (int)blockIdx.x;
+    double e = 0.0, s = 0.0;
+    for (int p = (int)threadIdx.x; p < kCropPix; p += 256) {
+        const size_t i = (size_t)n * kCropPix + p;
+        const double d = (double)a[i] - (double)b[i];
+        e += d * d;
+        s += (double)b[i] * (double)b[i];
+    }
+    e = pfh::wave_sum(e);
+    s = pfh::wave_sum(s);
+    if ((threadIdx.x & 63) == 0) {
+        r1[threadIdx.x >> 6] = e;
+        r2[threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double E = ((r1[0] + r1[1]) + r1[2]) + r1[3];
+        const double T = ((r2[0] + r2[1]) + r2[2]) + r2[3];
+        out[n] = T > 0.0 ? (float)sqrt(E / T) : (E > 0.0 ? 1e30f : 0.0f);
+    }
+}
+
+// Accepted cells: write the predicted crop into the tile's active window
+// (window pixels outside the crop become 0). Cells with accept[n] == 0 are
+// left alone.
+__global__ __launch_bounds__(256) void k_sur_writeback(float* phi, const pfh_cell_state* cells,
+                                                       const uint8_t* cls, const int* origin,
+                                                       const float* pred, const int* accept,
+                                                       int L) {
+    const int n = (int)blockIdx.x;
+    if (!accept[n]) return;
+    const pfh_cell_state c = cells[n];
+    const pfh::Shape s = pfh::shape_of((int)cls[n]);
+    float* tile = phi + (size_t)n * pfh::kTileArea;
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    for (int p = (int)threadIdx.x; p < s.wx * s.wy; p += (int)blockDim.x) {
+        const int a = p % s.wx, b = p / s.wx;
+        int wx = c.gx0 + a;
+        if (wx >= L) wx -= L;
+        int wy = c.gy0 + b;
+        if (wy >= L) wy -= L;
+        int i = wx - ox;
+        if (i < 0) i += L;
+        int j = wy - oy;
+        if (j < 0) j += L;
+        float v = 0.0f;
+        if (i < kCrop && j < kCrop) v = pred[(size_t)n * kCropPix + j * kCrop + i];
+        tile[(size_t)(s.ty0 + b) * pfh::kTilePitch + s.tx0 + a] = v;
+    }
+}
+
+// The polarity of K skipped steps: the plain step's tumble draw
+// (src/kernels.cu:98-114) for steps step0 .. step0+K-1, in order.
+__global__ void k_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+                             unsigned long long seed, double p_tumble) {
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n >= N) return;
+    pfh_cell_state* c = cells + n;
+    float theta = c->theta;
+    uint32_t ctr = c->tumble_ctr;
+    for (int k = 0; k < K; ++k) {
+        const unsigned long long step = step0 + (unsigned long long)k;
+        const pfh::Philox4 r = pfh::philox4x32_10(
+            (uint32_t)(step & 0xFFFFFFFFull), (uint32_t)(step >> 32), (uint32_t)c->global_id,
+            0xA5A5A5A5u, (uint32_t)(seed & 0xFFFFFFFFull), (uint32_t)(seed >> 32));
+        if (pfh::uniform53(r.v[0], r.v[1]) < p_tumble) {
+            theta = (float)(2.0 * pfh::kPi * pfh::uniform53(r.v[2], r.v[3]));
+            ++ctr;
+        }
+    }
+    c->theta = theta;
+    c->tumble_ctr = ctr;
+}
+
+}  // namespace sur
+
+// -------------------------------------------------------------------------
+// C ABI
+// -------------------------------------------------------------------------
+using namespace pfh_host;
+
+namespace {
+int sur_grid() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 512;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 512;
+    return 2 * (cus > 0 ? cus : 256);
+}
+}  // namespace
+
+extern "C" {
+
+int pfh_sur_crop(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                 const uint32_t* S, int N, int L, int P, uint16_t* X, float* phic,
+                 int* origin, void* stream) {
+    if (!phi || !cells || !cls || !S || !X || !phic || !origin || N <= 0 || L < sur::kCrop ||
+        P < L)
+        return invalid("sur_crop");
+    hipLaunchKernelGGL(sur::k_sur_crop, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
+                       phi, cells, cls, S, L, P, X, phic, origin);
+    return launch_status("k_sur_crop");
+}
+
+int pfh_sur_recrop(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                   const int* origin, int N, int L, float* out, void* stream) {
+    if (!phi || !cells || !cls || !origin || !out || N <= 0 || L < sur::kCrop)
+        return invalid("sur_recrop");
+    hipLaunchKernelGGL(sur::k_sur_recrop, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
+                       phi, cells, cls, origin, L, out);
+    return launch_status("k_sur_recrop");
+}
+
+int pfh_sur_conv(int cin, const uint16_t* X, const uint16_t* W, const float* bias,
+                 const uint16_t* mask, uint16_t* Y, int ncrops, int relu, void* stream) {
+    if (!X || !W || !Y || ncrops <= 0 || (cin != 8 && cin != 32)) return invalid("sur_conv");
+    const int grid = sur_grid();
+    if (cin == 8)
+        hipLaunchKernelGGL(sur::k_sur_conv<8>, dim3((unsigned)grid), dim3(256), 0,
+                           as_stream(stream), X, W, bias, mask, Y, ncrops, relu);
+    else
+        hipLaunchKernelGGL(sur::k_sur_conv<32>, dim3((unsigned)grid), dim3(256), 0,
+                           as_stream(stream), X, W, bias, mask, Y, ncrops, relu);
+    return launch_status("k_sur_conv");
+}
+
+int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic, float* dphi,
+                float* out, int ncrops, void* stream) {
+    if (!H || !W4 || ncrops <= 0 || (out && !phic)) return invalid("sur_out");
+    hipLaunchKernelGGL(sur::k_sur_out, dim3(2048), dim3(256), 0, as_stream(stream), H, W4, b4,
+                       phic, dphi, out, ncrops);
+    return launch_status("k_sur_out");
+}
+
+int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
+                 int ncrops, float scale, void* stream) {
+    if (!dphi || !target || !g || !crop_loss || ncrops <= 0) return invalid("sur_loss");
+    hipLaunchKernelGGL(sur::k_sur_loss, dim3((unsigned)ncrops), dim3(256), 0,
+                       as_stream(stream), dphi, target, g, crop_loss, scale);
+    return launch_status("k_sur_loss");
+}
+
+int pfh_sur_out_bwd(const float* g, const uint16_t* H, const float* W4, uint16_t* dH,
+                    float* dW4_part, int ncrops, void* stream) {
+    if (!g || !H || !W4 || !dH || !dW4_part || ncrops <= 0) return invalid("sur_out_bwd");
+    hipLaunchKernelGGL(sur::k_sur_out_bwd, dim3((unsigned)ncrops), dim3(256), 0,
+                       as_stream(stream), g, H, W4, dH, dW4_part);
+    return launch_status("k_sur_out_bwd");
+}
+
+int pfh_sur_wgrad(int cin, const uint16_t* dY, const uint16_t* X, float* part, int ncrops,
+                  void* stream) {
+    if (!dY || !X || !part || ncrops <= 0 || (cin != 8 && cin != 32)) return invalid("sur_wgrad");
+    if (cin == 8)
+        hipLaunchKernelGGL(sur::k_sur_wgrad<8>, dim3((unsigned)ncrops), dim3(256), 0,
+                           as_stream(stream), dY, X, part);
+    else
+        hipLaunchKernelGGL(sur::k_sur_wgrad<32>, dim3((unsigned)ncrops), dim3(256), 0,
+                           as_stream(stream), dY, X, part);
+    return launch_status("k_sur_wgrad");
+}
+
+int pfh_sur_reduce(const float* part, int ncrops, int width, float* out, void* stream) {
+    if (!part || !out || ncrops <= 0 || width <= 0) return invalid("sur_reduce");
+    hipLaunchKernelGGL(sur::k_sur_reduce, dim3((unsigned)((width + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), part, ncrops, width, out);
+    return launch_status("k_sur_reduce");
+}
+
+int pfh_sur_flip(const uint16_t* W, uint16_t* Wt, void* stream) {
+    if (!W || !Wt) return invalid("sur_flip");
+    hipLaunchKernelGGL(sur::k_sur_flip, dim3((32 * 9 * 32 + 255) / 256), dim3(256), 0,
+                       as_stream(stream), W, Wt);
+    return launch_status("k_sur_flip");
+}
+
+int pfh_sur_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int n, float lr,
+                 float b1, float b2, float eps, float c1, float c2, void* stream) {
+    if (!w || !g || !m || !v || n <= 0) return invalid("sur_adam");
+    hipLaunchKernelGGL(sur::k_sur_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), w, g, m, v, wb, n, lr, b1, b2, eps, c1, c2);
+    return launch_status("k_sur_adam");
+}
+
+int pfh_sur_rel_l2(const float* a, const float* b, float* out, int ncrops, void* stream) {
+    if (!a || !b || !out || ncrops <= 0) return invalid("sur_rel_l2");
+    hipLaunchKernelGGL(sur::k_sur_rel_l2, dim3((unsigned)ncrops), dim3(256), 0,
+                       as_stream(stream), a, b, out);
+    return launch_status("k_sur_rel_l2");
+}
+
+int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                      const int* origin, const float* pred, const int* accept, int N, int L,
+                      void* stream) {
+    if (!phi || !cells || !cls || !origin || !pred || !accept || N <= 0 || L < sur::kCrop)
+        return invalid("sur_writeback");
+    hipLaunchKernelGGL(sur::k_sur_writeback, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
+                       phi, cells, cls, origin, pred, accept, L);
+    return launch_status("k_sur_writeback");
+}
+
+int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+                   unsigned long long polarity_seed, double p_tumble, void* stream) {
+    if (!cells || N <= 0 || K < 0) return invalid("sur_tumble");
+    if (K == 0) return PFH_OK;
+    hipLaunchKernelGGL(sur::k_sur_tumble, dim3((unsigned)((N + 127) / 128)), dim3(128), 0,
+                       as_stream(stream), cells, N, step0, K, polarity_seed, p_tumble);
+    return launch_status("k_sur_tumble");
+}
+
+}  // extern "C"

@@ -176,6 +176,19 @@ SIGNATURES = {
     "pfh3d_calibrate_seed_radius": (_I, [C.c_double, C.c_double, _I, C.POINTER(C.c_double)]),
     "pfh3d_sequential_centres": (_I, [_I, C.c_double, C.c_double, C.c_uint64, _P, _P, _P]),
     "pfh3d_initial_polarity": (_I, [C.c_int64, C.c_uint64, _P]),
+    "pfh_sur_crop": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    "pfh_sur_recrop": (_I, [_P, _P, _P, _P, _I, _I, _P, _P]),
+    "pfh_sur_conv": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _P]),
+    "pfh_sur_out": (_I, [_P, _P, _F, _P, _P, _P, _I, _P]),
+    "pfh_sur_loss": (_I, [_P, _P, _P, _P, _I, _F, _P]),
+    "pfh_sur_out_bwd": (_I, [_P, _P, _P, _P, _P, _I, _P]),
+    "pfh_sur_wgrad": (_I, [_I, _P, _P, _P, _I, _P]),
+    "pfh_sur_reduce": (_I, [_P, _I, _I, _P, _P]),
+    "pfh_sur_flip": (_I, [_P, _P, _P]),
+    "pfh_sur_adam": (_I, [_P, _P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P]),
+    "pfh_sur_rel_l2": (_I, [_P, _P, _P, _I, _P]),
+    "pfh_sur_writeback": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P]),
+    "pfh_sur_tumble": (_I, [_P, _I, C.c_ulonglong, _I, C.c_ulonglong, C.c_double, _P]),
 }
 
 # Entry points the ABI header declares; tests check the library exports all.

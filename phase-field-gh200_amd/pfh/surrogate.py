@@ -1,0 +1,411 @@
+"""Learned surrogate of the 2D step: a BF16 CNN that advances every cell's
+96x96 crop by K plain steps, trained on crops the plain solver generates,
+with a plain check step per cell and a fallback to the plain method.
+
+Network (kernels in csrc/sur_kernels.hip): input 8 channels (phi,
+S_other = S - q(phi), cos theta, sin theta, 4 zero channels), three 3x3
+convolutions of 32 channels with ReLU on MFMA (BF16 operands, FP32
+accumulation), a 3x3 output convolution to one channel; the prediction is
+phi + scale * out. Training: mean squared error on (phi_{n+K} - phi_n) /
+scale, Adam on FP32 master weights, every gradient from the library's own
+backward kernels.
+
+Acceptance (the check-step rule of this build): for each cell, one explicit
+plain step of size K dt from phi_n (the plain kernel with dt' = K dt, run on
+copies of the state) gives a first-order reference of phi_{n+K}; a cell is
+accepted when the relative L2 distance between the prediction and that
+reference on its crop is at most `tol`. If every cell is accepted the
+predictions are written into the tiles and the aggregate S, the moments and
+the K skipped polarity draws are recomputed exactly (k_scatter_all /
+k_init_moments / the plain tumble rule); otherwise the block is advanced by
+K plain steps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import abi
+
+CROP = 96
+PIX = CROP * CROP
+C0 = 8
+CH = 32
+SIZES = {"W1": CH * 9 * C0, "b1": CH, "W2": CH * 9 * CH, "b2": CH, "W3": CH * 9 * CH,
+         "b3": CH, "W4": 9 * CH, "b4": 1}
+ORDER = list(SIZES)
+NPARAM = sum(SIZES.values())
+WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "surrogate_k8.npz"
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+class Net:
+    """Device parameters: one fp32 master vector (views per tensor), bf16
+    copies of the MFMA weights, Adam moments, the output scale."""
+
+    def __init__(self, device="cuda:0", seed: int = 0, scale: float = 1e-3):
+        self.dev = torch.device(device)
+        self.lib = abi.load()
+        self.w = torch.zeros(NPARAM, dtype=torch.float32, device=self.dev)
+        self.m = torch.zeros_like(self.w)
+        self.v = torch.zeros_like(self.w)
+        self.g = torch.zeros_like(self.w)
+        self.scale = float(scale)
+        self.t = 0
+        self.views = {}
+        self.gviews = {}
+        o = 0
+        for k in ORDER:
+            self.views[k] = self.w[o:o + SIZES[k]]
+            self.gviews[k] = self.g[o:o + SIZES[k]]
+            o += SIZES[k]
+        self.wb = {k: torch.zeros(SIZES[k], dtype=torch.int16, device=self.dev)
+                   for k in ("W1", "W2", "W3")}
+        self.wflip = {k: torch.zeros(SIZES[k], dtype=torch.int16, device=self.dev)
+                      for k in ("W2", "W3")}
+        self._init(seed)
+
+    def _init(self, seed: int) -> None:
+        rng = np.random.default_rng(seed)
+        host = {}
+        w1 = np.zeros((CH, 9, C0), np.float32)
+        w1[:, :, :4] = rng.normal(0.0, math.sqrt(2.0 / 36.0), (CH, 9, 4))
+        host["W1"] = w1
+        host["W2"] = rng.normal(0.0, math.sqrt(2.0 / 288.0), (CH, 9, CH)).astype(np.float32)
+        host["W3"] = rng.normal(0.0, math.sqrt(2.0 / 288.0), (CH, 9, CH)).astype(np.float32)
+        host["W4"] = np.zeros((9, CH), np.float32)
+        for k in ("b1", "b2", "b3", "b4"):
+            host[k] = np.zeros(SIZES[k], np.float32)
+        self.set_host(host)
+
+    # ---- host transfer ----
+    def set_host(self, host: dict) -> None:
+        for k in ORDER:
+            self.views[k].copy_(torch.from_numpy(np.ascontiguousarray(host[k], np.float32)
+                                                 .reshape(-1)))
+        self.refresh()
+
+    def host(self) -> dict:
+        return {k: self.views[k].cpu().numpy().copy() for k in ORDER}
+
+    def refresh(self) -> None:
+        """bf16 copies and the tap-flipped backward-data weights."""
+        for k in ("W1", "W2", "W3"):
+            self.wb[k].copy_(bf16_bits_torch(self.views[k]))
+        for k in ("W2", "W3"):
+            abi.check(self.lib.pfh_sur_flip(_ptr(self.wb[k]), _ptr(self.wflip[k]), None),
+                      "pfh_sur_flip")
+
+    def save(self, path: Path) -> None:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        h = self.host()
+        np.savez(path, scale=np.float64(self.scale), **h)
+
+    @classmethod
+    def load(cls, path: Path = WEIGHTS_PATH, device="cuda:0") -> "Net":
+        z = np.load(path, allow_pickle=False)
+        net = cls(device=device, scale=float(z["scale"]))
+        net.set_host({k: z[k] for k in ORDER})
+        return net
+
+    def b4(self) -> float:
+        return float(self.views["b4"].item())
+
+
+def bf16_bits_torch(x: torch.Tensor) -> torch.Tensor:
+    """binary32 -> bfloat16 bit patterns (round to nearest even) as int16."""
+    u = x.contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    return r.to(torch.int32).to(torch.int16)
+
+
+class Workspace:
+    """Activations and gradients for up to `n` crops."""
+
+    def __init__(self, n: int, device, train: bool = False):
+        z = dict(device=device)
+        self.n = n
+        self.X0 = torch.zeros(n * PIX * C0, dtype=torch.int16, **z)
+        self.h = [torch.zeros(n * PIX * CH, dtype=torch.int16, **z) for _ in range(3)]
+        self.phic = torch.zeros(n * PIX, dtype=torch.float32, **z)
+        self.dphi = torch.zeros(n * PIX, dtype=torch.float32, **z)
+        self.pred = torch.zeros(n * PIX, dtype=torch.float32, **z)
+        self.origin = torch.zeros(2 * n, dtype=torch.int32, **z)
+        if train:
+            self.g = torch.zeros(n * PIX, dtype=torch.float32, **z)
+            self.dh = [torch.zeros(n * PIX * CH, dtype=torch.int16, **z) for _ in range(3)]
+            self.crop_loss = torch.zeros(n, dtype=torch.float64, **z)
+            self.part = torch.zeros(n * (CH * 9 * CH + CH), dtype=torch.float32, **z)
+
+
+def forward(net: Net, ws: Workspace, n: int, X0=None, phic=None, pred=True, stream=None):
+    """dphi (and pred = phic + scale*out when pred) for n crops."""
+    lib = net.lib
+    X0 = ws.X0 if X0 is None else X0
+    phic = ws.phic if phic is None else phic
+    b = net.views
+    abi.check(lib.pfh_sur_conv(8, _ptr(X0), _ptr(net.wb["W1"]), _ptr(b["b1"]), None,
+                               _ptr(ws.h[0]), n, 1, stream), "conv1")
+    abi.check(lib.pfh_sur_conv(32, _ptr(ws.h[0]), _ptr(net.wb["W2"]), _ptr(b["b2"]), None,
+                               _ptr(ws.h[1]), n, 1, stream), "conv2")
+    abi.check(lib.pfh_sur_conv(32, _ptr(ws.h[1]), _ptr(net.wb["W3"]), _ptr(b["b3"]), None,
+                               _ptr(ws.h[2]), n, 1, stream), "conv3")
+    abi.check(lib.pfh_sur_out(_ptr(ws.h[2]), _ptr(b["W4"]), net.b4(), _ptr(phic),
+                              _ptr(ws.dphi), _ptr(ws.pred) if pred else None, n, stream),
+              "out")
+
+
+def backward(net: Net, ws: Workspace, n: int, target: torch.Tensor, X0=None, stream=None):
+    """Gradients into net.g for L = mean (dphi - target)^2 over n crops;
+    returns the loss as a device scalar."""
+    lib = net.lib
+    X0 = ws.X0 if X0 is None else X0
+    count = n * PIX
+    abi.check(lib.pfh_sur_loss(_ptr(ws.dphi), _ptr(target), _ptr(ws.g), _ptr(ws.crop_loss), n,
+                               2.0 / count, stream), "loss")
+    gv = net.gviews
+    # output layer: dh3 and dW4/db4 (partials laid out [n][289])
+    abi.check(lib.pfh_sur_out_bwd(_ptr(ws.g), _ptr(ws.h[2]), _ptr(net.views["W4"]),
+                                  _ptr(ws.dh[2]), _ptr(ws.part), n, stream), "out_bwd")
+    tmp = torch.empty(9 * CH + 1, dtype=torch.float32, device=net.dev)
+    abi.check(lib.pfh_sur_reduce(_ptr(ws.part), n, 9 * CH + 1, _ptr(tmp), stream), "reduce4")
+    gv["W4"].copy_(tmp[:9 * CH])
+    gv["b4"].copy_(tmp[9 * CH:])
+    # hidden layers 3, 2, 1
+    layers = [("W3", "b3", ws.h[1], 32), ("W2", "b2", ws.h[0], 32), ("W1", "b1", X0, 8)]
+    dys = [ws.dh[2], ws.dh[1], ws.dh[0]]
+    for i, (wk, bk, Xin, cin) in enumerate(layers):
+        dY = dys[i]
+        width = CH * 9 * cin + CH
+        abi.check(lib.pfh_sur_wgrad(cin, _ptr(dY), _ptr(Xin), _ptr(ws.part), n, stream), "wgrad")
+        red = torch.empty(width, dtype=torch.float32, device=net.dev)
+        abi.check(lib.pfh_sur_reduce(_ptr(ws.part), n, width, _ptr(red), stream), "reduce")
+        gv[wk].copy_(red[:CH * 9 * cin])
+        gv[bk].copy_(red[CH * 9 * cin:])
+        if cin == 32:
+            # dX = conv(dY, flipped W) masked by the layer input's ReLU
+            abi.check(lib.pfh_sur_conv(32, _ptr(dY), _ptr(net.wflip[wk]), None, _ptr(Xin),
+                                       _ptr(dys[i + 1]), n, 0, stream), "dgrad")
+    return ws.crop_loss[:n].sum() / count
+
+
+def adam_step(net: Net, lr: float, b1=0.9, b2=0.999, eps=1e-8) -> None:
+    net.t += 1
+    c1 = 1.0 - b1 ** net.t
+    c2 = 1.0 - b2 ** net.t
+    abi.check(net.lib.pfh_sur_adam(_ptr(net.w), _ptr(net.g), _ptr(net.m), _ptr(net.v), None,
+                                   NPARAM, lr, b1, b2, eps, c1, c2, None), "adam")
+    net.refresh()
+
+
+# ---------------------------------------------------------------------------
+# data from the plain solver
+# ---------------------------------------------------------------------------
+def crop_sim(sim, ws: Workspace, out_X0=None, out_phic=None, out_origin=None) -> None:
+    """Network inputs of every cell of a Sim2D at its current step."""
+    lib = abi.load()
+    s = sim.steps_done
+    X0 = ws.X0 if out_X0 is None else out_X0
+    phic = ws.phic if out_phic is None else out_phic
+    origin = ws.origin if out_origin is None else out_origin
+    abi.check(lib.pfh_sur_crop(_ptr(sim.phi[s % 2]), _ptr(sim.cell), _ptr(sim.cls),
+                               sim._s_ptr(s % 3), sim.N, sim.L, sim.P, _ptr(X0), _ptr(phic),
+                               _ptr(origin), sim._sp()), "pfh_sur_crop")
+
+
+def recrop_sim(sim, origin: torch.Tensor, out: torch.Tensor) -> None:
+    lib = abi.load()
+    s = sim.steps_done
+    abi.check(lib.pfh_sur_recrop(_ptr(sim.phi[s % 2]), _ptr(sim.cell), _ptr(sim.cls),
+                                 _ptr(origin), sim.N, sim.L, _ptr(out), sim._sp()),
+              "pfh_sur_recrop")
+
+
+def make_dataset(seeds, samples_per_seed: int = 2, K: int = 8, warm: int = 300,
+                 gap: int = 40, device="cuda:0", log=None):
+    """Training crops from 2048^2 / 1024-cell monolayers (seeds as given):
+    after `warm` plain steps, `samples_per_seed` samples `gap` steps apart;
+    each sample is every cell's input crop at step n and the increment
+    phi_{n+K} - phi_n on the same window."""
+    from . import params, sim2d
+    per = 1024 * samples_per_seed * len(seeds)
+    dev = torch.device(device)
+    X = torch.zeros(per * PIX * C0, dtype=torch.int16, device=dev)
+    D = torch.zeros(per * PIX, dtype=torch.float32, device=dev)
+    ws = Workspace(1024, dev)
+    after = torch.zeros(1024 * PIX, dtype=torch.float32, device=dev)
+    k = 0
+    for seed in seeds:
+        wl = params.monolayer(1024, 2048, 20.0, 32, 4.0, int(seed), name=f"train_{seed}")
+        sim = sim2d.Sim2D(wl, device=device)
+        sim.capture_graph()
+        sim.run(warm)
+        for s in range(samples_per_seed):
+            if s:
+                sim.run(gap)
+            o = k * 1024
+            crop_sim(sim, ws, out_X0=X[o * PIX * C0:(o + 1024) * PIX * C0])
+            sim.run(K)
+            recrop_sim(sim, ws.origin, after)
+            D[o * PIX:(o + 1024) * PIX].copy_(after - ws.phic)
+            k += 1
+        if sim.fatal():
+            raise RuntimeError(f"fatal flags in training run seed {seed}")
+        del sim
+        if log:
+            log(f"dataset: seed {seed} done ({k * 1024} crops)")
+    return X, D
+
+
+def train(net: Net, X: torch.Tensor, D: torch.Tensor, epochs: int = 4, batch: int = 128,
+          lr: float = 1e-3, seed: int = 0, log=None) -> list:
+    """Adam over shuffled minibatches of (input, increment/scale)."""
+    n = D.numel() // PIX
+    ws = Workspace(batch, net.dev, train=True)
+    tgt = torch.zeros(batch * PIX, dtype=torch.float32, device=net.dev)
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    Xv = X.view(n, PIX * C0)
+    Dv = D.view(n, PIX)
+    steps_total = epochs * (n // batch)
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=gen).to(net.dev)
+        tot = 0.0
+        for bi in range(n // batch):
+            idx = perm[bi * batch:(bi + 1) * batch]
+            ws.X0.view(batch, PIX * C0).copy_(Xv[idx])
+            tgt.view(batch, PIX).copy_(Dv[idx] / net.scale)
+            forward(net, ws, batch, pred=False)
+            loss = backward(net, ws, batch, tgt)
+            # cosine decay
+            cur = lr * 0.5 * (1.0 + math.cos(math.pi * step / max(1, steps_total)))
+            adam_step(net, cur)
+            step += 1
+            tot += float(loss)
+        hist.append(tot / max(1, n // batch))
+        if log:
+            log(f"epoch {ep}: mean loss {hist[-1]:.4e}")
+    return hist
+
+
+def deploy_weights(net: Net) -> tuple[torch.Tensor, float]:
+    """Output-layer weights with the scale folded in (fp32)."""
+    return net.views["W4"] * net.scale, net.b4() * net.scale
+
+
+# ---------------------------------------------------------------------------
+# deployment on a Sim2D
+# ---------------------------------------------------------------------------
+class SurrogateRunner:
+    """Advances a Sim2D K steps at a time with the surrogate, checked per
+    cell against one explicit step of size K dt, falling back to K plain
+    steps when any cell is rejected."""
+
+    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3):
+        self.sim, self.net, self.K, self.tol = sim, net, K, tol
+        self.lib = abi.load()
+        dev = sim.dev
+        N = sim.N
+        self.ws = Workspace(N, dev)
+        self.W4d, self.b4d = deploy_weights(net)
+        self.phi_chk = torch.zeros_like(sim.phi[0])
+        self.S_chk = torch.zeros(2 * sim.words, dtype=torch.int32, device=dev)
+        self.cell_chk = torch.zeros_like(sim.cell)
+        self.cls_chk = torch.zeros_like(sim.cls)
+        self.cur_chk = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.step_chk = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.flags_chk = torch.zeros_like(sim.flags)
+        self.chk = torch.zeros(N * PIX, dtype=torch.float32, device=dev)
+        self.err = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.accept = torch.ones(N, dtype=torch.int32, device=dev)
+        self.blocks = 0
+        self.fallbacks = 0
+
+    def _check_args(self, slot: int) -> abi.StepArgs:
+        sim = self.sim
+        a = abi.StepArgs()
+        C.memmove(C.addressof(a), C.addressof(sim._args[slot % 6]), C.sizeof(a))
+        a.phi_out = _ptr(self.phi_chk)
+        a.S_sc = _ptr(self.S_chk)
+        a.S_cl = _ptr(self.S_chk) + 4 * sim.words
+        a.clear_ahead_words = 0
+        a.cell = _ptr(self.cell_chk)
+        a.cell_cls = _ptr(self.cls_chk)
+        a.cursor_use = _ptr(self.cur_chk)
+        a.cursor_clear = _ptr(self.cur_chk) + 8
+        a.step_rd = _ptr(self.step_chk)
+        a.step_wr = _ptr(self.step_chk) + 8
+        a.flags = _ptr(self.flags_chk)
+        a.dt = float(np.float32(sim.p.dt * self.K))
+        a.physical_dt = sim.p.dt * self.K
+        return a
+
+    def block(self) -> bool:
+        """Advance K steps; returns True when the surrogate was accepted."""
+        sim, lib, K = self.sim, self.lib, self.K
+        s = sim.steps_done
+        sp = sim._sp()
+        N = sim.N
+        crop_sim(sim, self.ws)
+        b = self.net.views
+        abi.check(lib.pfh_sur_conv(8, _ptr(self.ws.X0), _ptr(self.net.wb["W1"]), _ptr(b["b1"]),
+                                   None, _ptr(self.ws.h[0]), N, 1, sp), "conv1")
+        abi.check(lib.pfh_sur_conv(32, _ptr(self.ws.h[0]), _ptr(self.net.wb["W2"]),
+                                   _ptr(b["b2"]), None, _ptr(self.ws.h[1]), N, 1, sp), "conv2")
+        abi.check(lib.pfh_sur_conv(32, _ptr(self.ws.h[1]), _ptr(self.net.wb["W3"]),
+                                   _ptr(b["b3"]), None, _ptr(self.ws.h[2]), N, 1, sp), "conv3")
+        abi.check(lib.pfh_sur_out(_ptr(self.ws.h[2]), _ptr(self.W4d), self.b4d,
+                                  _ptr(self.ws.phic), _ptr(self.ws.dphi), _ptr(self.ws.pred), N,
+                                  sp), "out")
+        # check step: one plain step of K dt from phi_n on copies of the state
+        self.cell_chk.copy_(sim.cell)
+        self.cls_chk.copy_(sim.cls)
+        self.S_chk[:sim.words].zero_()
+        self.cur_chk.zero_()
+        self.step_chk.fill_(s)
+        a = self._check_args(s)
+        abi.check(lib.pfh2d_launch_step(C.byref(a), sim.grid, sp, None, 0, 0.0, 0), "check step")
+        abi.check(lib.pfh_sur_recrop(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
+                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.chk), sp),
+                  "recrop check")
+        abi.check(lib.pfh_sur_rel_l2(_ptr(self.ws.pred), _ptr(self.chk), _ptr(self.err), N, sp),
+                  "rel_l2")
+        ok = bool((self.err <= self.tol).all().item())
+        self.blocks += 1
+        if not ok:
+            self.fallbacks += 1
+            sim.run(K)
+            return False
+        pin = s % 2
+        self.accept.fill_(1)
+        abi.check(lib.pfh_sur_writeback(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
+                                        _ptr(self.ws.origin), _ptr(self.ws.pred),
+                                        _ptr(self.accept), N, sim.L, sp), "writeback")
+        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, s, K, sim.p.polarity_stream(),
+                                     sim.p.p_tumble(), sp), "tumble")
+        e = s + K
+        sim.S.zero_()
+        abi.check(lib.pfh2d_scatter_all(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
+                                        sim._s_ptr(e % 3), N, sim.L, sim.P, _ptr(sim.flags), sp),
+                  "scatter_all")
+        abi.check(lib.pfh2d_init_moments(_ptr(sim.cell), _ptr(sim.cls), _ptr(sim.phi[pin]), N,
+                                         sp), "init_moments")
+        pin_e = e % 2
+        sim.stepc.copy_(torch.tensor([e if pin_e == 0 else e + 1, e if pin_e == 1 else e + 1],
+                                     dtype=torch.int64))
+        sim.cursor.zero_()
+        sim.steps_done = e
+        return True
+
+    def run(self, steps: int) -> None:
+        assert steps % self.K == 0
+        for _ in range(steps // self.K):
+            self.block()

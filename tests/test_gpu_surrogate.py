@@ -1,0 +1,175 @@
+"""GPU checks of the surrogate kernels against the numpy oracle
+(oracle/sur_ref.py): the crop generator bit for bit, the BF16 MFMA forward
+pass (hidden activations within two bfloat16 ulps on at most 0.1% of the
+elements, output within float32 accumulation error), every gradient of the backward pass (relative
+Frobenius error), that training on solver crops lowers the loss, and, with
+the committed weights, the accepted surrogate blocks against the plain
+method on the 2048^2 / 1024-cell monolayer.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from pfh import params, sim2d, surrogate as su
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-surrogate] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def R():
+    from oracle import sur_ref
+    return sur_ref
+
+
+@pytest.fixture(scope="module")
+def c1_after():
+    wl = params.workload_c1()
+    sim = sim2d.Sim2D(wl)
+    sim.step(5)
+    sim.sync()
+    return sim
+
+
+def _bits(t, shape):
+    return t.cpu().numpy().view(np.uint16).reshape(shape)
+
+
+def test_crop_matches_oracle(R, oracle, c1_after):
+    sim = c1_after
+    say("crops of C1 after 5 steps")
+    ws = su.Workspace(sim.N, sim.dev)
+    su.crop_sim(sim, ws)
+    sim.sync()
+    X, phic, origin = R.crops(sim.phi_now(), sim.cells(), sim.classes(), sim.S_now(), sim.L,
+                              sim.P, oracle.sincosf)
+    gX = _bits(ws.X0, (sim.N, 96, 96, 8))
+    assert np.array_equal(gX, R.bf16_bits(X))
+    assert np.array_equal(ws.phic.cpu().numpy().reshape(sim.N, 96, 96).view(np.uint32),
+                          phic.view(np.uint32))
+    assert np.array_equal(ws.origin.cpu().numpy().reshape(-1, 2), origin)
+    assert phic.max() > 0.9
+
+
+def _random_net(seed):
+    net = su.Net(seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    h = net.host()
+    for k in ("b1", "b2", "b3"):
+        h[k] = rng.normal(0, 0.05, h[k].shape).astype(np.float32)
+    h["W4"] = rng.normal(0, 0.2, h["W4"].shape).astype(np.float32)
+    h["b4"] = np.array([0.01], np.float32)
+    net.set_host(h)
+    return net
+
+
+def _oracle_params(R, net):
+    h = net.host()
+    return R.Params(h["W1"].reshape(32, 9, 8), h["b1"], h["W2"].reshape(32, 9, 32), h["b2"],
+                    h["W3"].reshape(32, 9, 32), h["b3"], h["W4"].reshape(9, 32), h["b4"][0])
+
+
+def _gpu_forward_values(R, ws, n, N):
+    """The kernels' own activations as float64 values."""
+    fw = {"X0": R.bf16_value(_bits(ws.X0, (N, 96, 96, 8))[:n]).astype(np.float64)}
+    for i, key in enumerate(("h1", "h2", "h3")):
+        fw[key] = R.bf16_value(_bits(ws.h[i], (N, 96, 96, 32))[:n]).astype(np.float64)
+    fw["dphi"] = ws.dphi.cpu().numpy().reshape(N, 96, 96)[:n].astype(np.float64)
+    return fw
+
+
+def test_forward_matches_oracle(R, c1_after):
+    """Each layer of the kernels against the oracle applied to the kernels'
+    own input of that layer."""
+    sim = c1_after
+    n = 4
+    say("forward pass, 4 crops, random weights, layer by layer")
+    ws = su.Workspace(sim.N, sim.dev)
+    su.crop_sim(sim, ws)
+    net = _random_net(11)
+    su.forward(net, ws, n)
+    torch.cuda.synchronize()
+    p = _oracle_params(R, net)
+    fw = _gpu_forward_values(R, ws, n, sim.N)
+    prev = fw["X0"]
+    for key, W, b in (("h1", p.W1, p.b1), ("h2", p.W2, p.b2), ("h3", p.W3, p.b3)):
+        o = R.bf16_bits(np.maximum(R.conv(prev, W, b), 0.0)).astype(np.int32)
+        g = R.bf16_bits(fw[key]).astype(np.int32)
+        diff = np.abs(g - o)
+        say(f"{key}: {np.count_nonzero(diff)} of {diff.size} differ, at most {diff.max()} "
+            f"bf16 ulp")
+        # fp32 (kernel) against fp64 (oracle) accumulation: sums that cancel
+        # to near zero can round to neighbouring bf16 values
+        assert diff.max() <= 2 and np.count_nonzero(diff) <= 1e-3 * diff.size
+        assert np.count_nonzero(o) > 0.1 * o.size
+        prev = fw[key]
+    ref = R.conv(fw["h3"], p.W4)[..., 0] + p.b4
+    err = np.abs(fw["dphi"] - ref).max()
+    say(f"dphi max abs error {err:.3e} (|dphi| max {np.abs(ref).max():.3e})")
+    assert err <= 1e-5 * np.abs(ref).max() + 1e-7
+    # end to end, the oracle's own forward pass stays close
+    full = R.forward(fw["X0"], p)
+    rel = np.linalg.norm(full["dphi"] - fw["dphi"]) / np.linalg.norm(full["dphi"])
+    say(f"end-to-end dphi relative L2 {rel:.2e}")
+    assert rel <= 1e-2
+
+
+def test_backward_matches_oracle(R, c1_after):
+    """Every gradient against the oracle's backward pass over the kernels'
+    own forward activations."""
+    sim = c1_after
+    n = 2
+    say("backward pass, 2 crops, random weights")
+    ws = su.Workspace(n, sim.dev, train=True)
+    full = su.Workspace(sim.N, sim.dev)
+    su.crop_sim(sim, full)
+    ws.X0.copy_(full.X0[:n * su.PIX * su.C0])
+    net = _random_net(5)
+    rng = np.random.default_rng(2)
+    target = rng.normal(0, 0.5, (n, 96, 96)).astype(np.float32)
+    tg = torch.from_numpy(target.reshape(-1)).to(sim.dev)
+    su.forward(net, ws, n, pred=False)
+    loss = float(su.backward(net, ws, n, tg))
+    torch.cuda.synchronize()
+    p = _oracle_params(R, net)
+    fw = _gpu_forward_values(R, ws, n, n)
+    g = R.backward(fw, target.astype(np.float64), p)
+    assert abs(loss - g["loss"]) <= 1e-5 * g["loss"]
+    gv = {k: v.cpu().numpy() for k, v in net.gviews.items()}
+    for k in ("W4", "b4", "W3", "b3", "W2", "b2", "W1", "b1"):
+        ref = np.asarray(g[k], np.float64).reshape(-1)
+        got = gv[k].astype(np.float64).reshape(-1)
+        if k == "W1":   # the four zero input channels carry no information
+            ref = ref.reshape(32, 9, 8)[:, :, :4].reshape(-1)
+            got = got.reshape(32, 9, 8)[:, :, :4].reshape(-1)
+        rel = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+        say(f"grad {k}: relative error {rel:.2e}")
+        assert rel <= 1e-2, k
+
+
+def test_training_lowers_loss():
+    say("training on C2-style crops: loss decreases")
+    X, D = su.make_dataset([1000], samples_per_seed=1, warm=100)
+    net = su.Net(seed=0)
+    n = D.numel() // su.PIX
+    held = 128
+    Xt, Dt = X[:(n - held) * su.PIX * su.C0], D[:(n - held) * su.PIX]
+    hist = su.train(net, Xt, Dt, epochs=10, batch=32, lr=2e-3)
+    say(f"loss history {hist}")
+    # held-out crops: the network's increment beats predicting no change
+    ws = su.Workspace(held, net.dev)
+    ws.X0.copy_(X[(n - held) * su.PIX * su.C0:])
+    su.forward(net, ws, held, pred=False)
+    d_true = D[(n - held) * su.PIX:].double()
+    d_net = ws.dphi.double() * net.scale
+    e_net = float(torch.linalg.norm(d_net - d_true))
+    e_id = float(torch.linalg.norm(d_true))
+    say(f"held-out increment error: network {e_net:.4e}, no-change {e_id:.4e}")
+    assert hist[-1] < 0.7 * hist[0]
+    assert e_net < 0.7 * e_id

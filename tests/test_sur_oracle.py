@@ -1,0 +1,72 @@
+"""Pins the surrogate network's CPU oracle (oracle/sur_ref.py): its
+backward pass against central finite differences in float64, the
+bfloat16 rounding rule against known bit patterns, and the backward-data
+weight flip against the direct adjoint definition."""
+import numpy as np
+import pytest
+
+from oracle import sur_ref as R
+
+
+def test_bf16_rounding_known_patterns():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5, 0.0,
+                  3.0e38, 1e-40], np.float32)
+    bits = R.bf16_bits(x)
+    # ties go to even: 1 + 2^-8 -> 1.0 (0x3F80), 1 + 3*2^-8 -> 1 + 2^-6 (0x3F82)
+    assert bits.tolist()[:6] == [0x3F80, 0x3F80, 0x3F81, 0x3F82, 0xC020, 0x0000]
+    assert R.bf16_value(bits)[2] == np.float32(1.0 + 2 ** -7)
+    assert np.isnan(R.bf16_value(R.bf16_bits(np.array([np.nan], np.float32))))[0]
+
+
+def _tiny(rng, H=7, Wd=6, n=2):
+    p = R.Params(rng.normal(0, 0.4, (32, 9, 8)), rng.normal(0, 0.1, 32),
+                 rng.normal(0, 0.1, (32, 9, 32)), rng.normal(0, 0.1, 32),
+                 rng.normal(0, 0.1, (32, 9, 32)), rng.normal(0, 0.1, 32),
+                 rng.normal(0, 0.3, (9, 32)), 0.05, bf16=False)
+    X = rng.normal(0, 1.0, (n, H, Wd, 8))
+    T = rng.normal(0, 0.5, (n, H, Wd))
+    return p, X, T
+
+
+def _loss(p, X, T):
+    return float(np.sum((R.forward(X, p)["dphi"] - T) ** 2) / T.size)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    p, X, T = _tiny(rng)
+    g = R.backward(R.forward(X, p), T, p)
+    h = 1e-6
+    for name, idx in (("W1", (3, 4, 1)), ("W2", (7, 0, 5)), ("W3", (11, 8, 30)),
+                      ("W4", (0, 2, 9)), ("b1", (5,)), ("b3", (17,))):
+        arr = getattr(p, name)
+        old = arr[idx]
+        arr[idx] = old + h
+        lp = _loss(p, X, T)
+        arr[idx] = old - h
+        lm = _loss(p, X, T)
+        arr[idx] = old
+        num = (lp - lm) / (2 * h)
+        ana = g[name][idx[1:]] if name == "W4" else g[name][idx]
+        assert abs(num - ana) <= 1e-6 * (1 + abs(num)), (name, num, ana)
+    p.b4 += h
+    lp = _loss(p, X, T)
+    p.b4 -= 2 * h
+    lm = _loss(p, X, T)
+    p.b4 += h
+    assert abs((lp - lm) / (2 * h) - g["b4"]) <= 1e-6
+
+
+def test_dgrad_is_the_adjoint_of_conv():
+    rng = np.random.default_rng(6)
+    X = rng.normal(size=(1, 5, 6, 32))
+    Y = rng.normal(size=(1, 5, 6, 32))
+    W = rng.normal(size=(32, 9, 32))
+    # <conv(X, W), Y> == <X, dgrad(Y, W)>
+    assert np.isclose(np.sum(R.conv(X, W) * Y), np.sum(X * R.conv_dgrad(Y, W)), rtol=1e-12)
+    # the flip the kernels use: Wt[ci][t][co] = W[co][8 - t][ci]
+    Wt = np.zeros_like(W)
+    for ci in range(32):
+        for t in range(9):
+            Wt[ci, t, :] = W[:, 8 - t, ci]
+    assert np.allclose(R.conv_dgrad(Y, W), R.conv(Y, Wt))
