@@ -35,6 +35,7 @@ for _p in (REPO, REPO / "phase-field-gh200_amd"):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
+BF16_PEAK_TFLOPS = 2500.0       # MI355X published dense BF16 MFMA rate
 TRAFFIC_SUMMARY = REPO / "profiles" / "pf2d_step_L2048_N1024_traffic.md"
 
 
@@ -173,6 +174,24 @@ def bench_3d(steps: int, warmup: int, with_cpu: bool) -> dict:
     return out
 
 
+def bench_surrogate(blocks: int) -> dict | None:
+    """Surrogate + fallback against the plain method on the same 2048^2 /
+    1024-cell state (8-step blocks, check rule and tolerance of
+    pfh.surrogate.SurrogateRunner), with the committed weights."""
+    from pfh import surrogate as su
+    from pfh import surrogate_train as st
+    if not su.WEIGHTS_PATH.exists():
+        return {"error": f"no weights at {su.WEIGHTS_PATH.name}"}
+    net = su.Net.load(su.WEIGHTS_PATH)
+    r = st.evaluate(net, blocks=blocks)
+    inf = r["infer"]
+    r["roofline"] = {"bound": "mfma", "kernel": inf["kernel"], "achieved": inf["tflops"],
+                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": inf["tflops"] / BF16_PEAK_TFLOPS,
+                     "flop_per_launch": inf["flop_per_launch"], "us_per_launch": inf["us"]}
+    return r
+
+
 def calibrate_copy(mib: int) -> int:
     import ctypes as C
 
@@ -201,6 +220,8 @@ def main() -> int:
                     help="2d: the headline 2048^2 step only; 3d: the 256^3 step only")
     ap.add_argument("--steps3d", type=int, default=600)
     ap.add_argument("--warmup3d", type=int, default=40)
+    ap.add_argument("--sur-blocks", type=int, default=25,
+                    help="8-step blocks of the surrogate A/B run (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 8 plain copies of MIB MiB "
                          "(the known-size kernel that scales hardware byte counters)")
@@ -280,6 +301,9 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(wl)
     del sim
     torch.cuda.empty_cache()
+    if args.sur_blocks > 0:
+        out["surrogate"] = bench_surrogate(args.sur_blocks)
+        torch.cuda.empty_cache()
     if args.workload == "all":
         out["pf3d"] = bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)

@@ -433,6 +433,13 @@ int pfh_sur_conv(int cin, const uint16_t* X, const uint16_t* W, const float* bia
  * may be NULL). */
 int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic, float* dphi,
                 float* out, int ncrops, void* stream);
+/* Whole-network inference in one kernel (activations stay in LDS): X as for
+ * pfh_sur_conv (8 channels, the first 4 used), W1 [32][9][8], W2 and W3
+ * [32][9][32] bf16, biases fp32, W4 [9][32] fp32 (applied as bf16 hi + lo).
+ * pred = phic + dphi; dphi may be NULL. X, W2 and W3 16-byte aligned. */
+int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
+                  const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
+                  const float* phic, float* dphi, float* pred, int ncrops, void* stream);
 /* Squared-error loss per crop (fp64) and g = scale (dphi - target). */
 int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
                  int ncrops, float scale, void* stream);
@@ -458,6 +465,16 @@ int pfh_sur_rel_l2(const float* a, const float* b, float* out, int ncrops, void*
 int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cls,
                       const int* origin, const float* pred, const int* accept, int N, int L,
                       void* stream);
+/* Acceptance test: err[n] = relative L2 of pred against the check step's
+ * field on the recorded crop windows; rejected[0] += cells with err > tol. */
+int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
+                  const int* origin, int N, int L, const float* pred, float tol, float* err,
+                  uint32_t* rejected, void* stream);
+/* Accepted block: prediction into every cell's window, q(phi) scattered into
+ * the zeroed aggregate S, moments as pfh2d_init_moments computes them. */
+int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,
+                   const float* pred, int N, int L, int P, uint32_t* S, uint32_t* flags,
+                   void* stream);
 /* The polarity draws of K skipped plain steps, steps step0 .. step0+K-1
  * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. */
 int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,

@@ -959,72 +959,19 @@ __global__ void k_init_tiles(float* phi_a, float* phi_b, pfh_cell_state* cell,
     }
 }
 
-// Moments of the stored field with the update's thread mapping: column bands
-// for the LDS classes, flat pixel index for classes 4-5.
+// Moments of the stored field with the update's thread mapping (column bands
+// for the LDS classes, flat pixel index for classes 4-5).
 __global__ __launch_bounds__(kNT) void k_init_moments(pfh_cell_state* cell,
                                                       const uint8_t* clsv,
                                                       const float* phi, int N) {
-    __shared__ double red[kWaves * 4];
-    __shared__ int ired[kWaves * 8];
     const int n = blockIdx.x;
     if (n >= N) return;
     const int cls = (int)clsv[n];
     const Shape sc = shape_of(cls);
-    const int wx = sc.wx, wy = sc.wy;
     const float* t = phi + (size_t)n * kTileArea + (size_t)sc.ty0 * kTilePitch + sc.tx0;
-    const int tid = (int)threadIdx.x;
-    double aV = 0.0, aCx = 0.0, aCy = 0.0;
-    int blox = wx, bhix = -1, bloy = wy, bhiy = -1, pmaxb = 0;
-    auto visit = [&](int a, int b) {
-        const float pn = t[(size_t)b * kTilePitch + a];
-        const double d = (double)pn * (double)pn;
-        aV += d;
-        aCx += d * (double)a;
-        aCy += d * (double)b;
-        pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
-        if (pn > kSupportEps) {
-            blox = min(blox, a); bhix = max(bhix, a);
-            bloy = min(bloy, b); bhiy = max(bhiy, b);
-        }
-    };
-    if (cls <= kClassStagedMax) {
-        const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
-        const int x = tid % wx, band = tid / wx;
-        if (band < nb)
-            for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);
-    } else {
-        for (int p = tid; p < wx * wy; p += kNT) visit(p % wx, p / wx);
-    }
-    const int lane = tid & (kWave - 1), wave = tid / kWave;
-    aV = wave_sum(aV); aCx = wave_sum(aCx); aCy = wave_sum(aCy);
-    blox = wave_min(blox); bhix = wave_max(bhix);
-    bloy = wave_min(bloy); bhiy = wave_max(bhiy);
-    pmaxb = wave_max(pmaxb);
-    if (lane == 0) {
-        red[wave * 4 + 0] = aV; red[wave * 4 + 1] = aCx; red[wave * 4 + 2] = aCy;
-        ired[wave * 8 + 0] = blox; ired[wave * 8 + 1] = bhix;
-        ired[wave * 8 + 2] = bloy; ired[wave * 8 + 3] = bhiy;
-        ired[wave * 8 + 4] = pmaxb;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double sV = 0.0, sCx = 0.0, sCy = 0.0;
-        int Blox = wx, Bhix = -1, Bloy = wy, Bhiy = -1, Pmax = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            sV += red[w * 4 + 0]; sCx += red[w * 4 + 1]; sCy += red[w * 4 + 2];
-            Blox = min(Blox, ired[w * 8 + 0]); Bhix = max(Bhix, ired[w * 8 + 1]);
-            Bloy = min(Bloy, ired[w * 8 + 2]); Bhiy = max(Bhiy, ired[w * 8 + 3]);
-            Pmax = max(Pmax, ired[w * 8 + 4]);
-        }
-        pfh_cell_state* cs = &cell[n];
-        cs->V = sV; cs->Cx = sCx; cs->Cy = sCy;
-        cs->bb_lo_x = Blox; cs->bb_hi_x = Bhix;
-        cs->bb_lo_y = Bloy; cs->bb_hi_y = Bhiy;
-        cs->phi_max = __uint_as_float((uint32_t)Pmax);
-        cs->promote_ctr = 0u;
-        cs->Ix = 0.0;
-        cs->Iy = 0.0;
-    }
+    const WindowMoments m =
+        window_moments(cls, [&](int a, int b) { return t[(size_t)b * kTilePitch + a]; });
+    if (threadIdx.x == 0) store_moments(&cell[n], m);
 }
 
 __global__ void k_scatter_all(const float* phi, const pfh_cell_state* cell,

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pfh.h"
+
 namespace pfh {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -232,6 +234,82 @@ __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_down(v, d, kWave));
     return v;
+}
+
+// ---- moments of one cell's window with the update's thread mapping ----
+// Column bands of threads (x = tid % wx, band = tid / wx, contiguous rows per
+// band) for the LDS classes, flat pixel index for classes 4-5; per-thread
+// fp64 sums in visiting order, the lane tree, then waves ascending from 0.
+struct WindowMoments {
+    double V, Cx, Cy;
+    int blox, bhix, bloy, bhiy, pmaxb;
+};
+
+// phi_at(a, b) returns phi at window pixel (a, b); every pixel is visited
+// exactly once, so phi_at may also store or scatter. Workgroup of kNT
+// threads; the result is valid in thread 0.
+template <class F>
+__device__ WindowMoments window_moments(int cls, F phi_at) {
+    __shared__ double red[kWaves * 3];
+    __shared__ int ired[kWaves * 5];
+    const Shape sc = shape_of(cls);
+    const int wx = sc.wx, wy = sc.wy;
+    const int tid = (int)threadIdx.x;
+    double aV = 0.0, aCx = 0.0, aCy = 0.0;
+    int blox = wx, bhix = -1, bloy = wy, bhiy = -1, pmaxb = 0;
+    auto visit = [&](int a, int b) {
+        const float pn = phi_at(a, b);
+        const double d = (double)pn * (double)pn;
+        aV += d;
+        aCx += d * (double)a;
+        aCy += d * (double)b;
+        pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
+        if (pn > kSupportEps) {
+            blox = min(blox, a); bhix = max(bhix, a);
+            bloy = min(bloy, b); bhiy = max(bhiy, b);
+        }
+    };
+    if (cls <= kClassStagedMax) {
+        const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
+        const int x = tid % wx, band = tid / wx;
+        if (band < nb)
+            for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);
+    } else {
+        for (int p = tid; p < wx * wy; p += kNT) visit(p % wx, p / wx);
+    }
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
+    aV = wave_sum(aV); aCx = wave_sum(aCx); aCy = wave_sum(aCy);
+    blox = wave_min(blox); bhix = wave_max(bhix);
+    bloy = wave_min(bloy); bhiy = wave_max(bhiy);
+    pmaxb = wave_max(pmaxb);
+    if (lane == 0) {
+        red[wave * 3 + 0] = aV; red[wave * 3 + 1] = aCx; red[wave * 3 + 2] = aCy;
+        ired[wave * 5 + 0] = blox; ired[wave * 5 + 1] = bhix;
+        ired[wave * 5 + 2] = bloy; ired[wave * 5 + 3] = bhiy;
+        ired[wave * 5 + 4] = pmaxb;
+    }
+    __syncthreads();
+    WindowMoments m = {0.0, 0.0, 0.0, wx, -1, wy, -1, 0};
+    if (tid == 0) {
+        for (int w = 0; w < kWaves; ++w) {
+            m.V += red[w * 3 + 0]; m.Cx += red[w * 3 + 1]; m.Cy += red[w * 3 + 2];
+            m.blox = min(m.blox, ired[w * 5 + 0]); m.bhix = max(m.bhix, ired[w * 5 + 1]);
+            m.bloy = min(m.bloy, ired[w * 5 + 2]); m.bhiy = max(m.bhiy, ired[w * 5 + 3]);
+            m.pmaxb = max(m.pmaxb, ired[w * 5 + 4]);
+        }
+    }
+    return m;
+}
+
+// Moments into the cell state; the interaction integral restarts at zero.
+__device__ __forceinline__ void store_moments(pfh_cell_state* cs, const WindowMoments& m) {
+    cs->V = m.V; cs->Cx = m.Cx; cs->Cy = m.Cy;
+    cs->bb_lo_x = m.blox; cs->bb_hi_x = m.bhix;
+    cs->bb_lo_y = m.bloy; cs->bb_hi_y = m.bhiy;
+    cs->phi_max = __uint_as_float((uint32_t)m.pmaxb);
+    cs->promote_ctr = 0u;
+    cs->Ix = 0.0;
+    cs->Iy = 0.0;
 }
 
 }  // namespace pfh

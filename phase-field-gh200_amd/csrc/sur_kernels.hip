@@ -243,34 +243,335 @@ __global__ __launch_bounds__(256) void k_sur_conv(const bf16_t* X, const bf16_t*
 // ---------------------------------------------------------------------------
 // output layer (32 -> 1), loss and their gradients (VALU, fp32)
 // ---------------------------------------------------------------------------
-// dphi = b4 + sum_{tap, ci} W4[tap][ci] h3[p + off(tap)][ci]; out = phic + dphi
+// dphi = b4 + sum_{tap, ci} W4[tap][ci] h3[p + off(tap)][ci]; out = phic + dphi.
+// Persistent over (crop, 4-row block) items like k_sur_conv: the 6 x 98
+// halo tile of h3 is staged in LDS, each lane then forms 288-term dot
+// products in fp32 (taps in order, channels in order).
 __global__ __launch_bounds__(256) void k_sur_out(const bf16_t* H, const float* W4, float b4,
                                                  const float* phic, float* dphi, float* out,
                                                  int ncrops) {
+    constexpr int PIXP = kCH + 8;
     __shared__ float w[9 * kCH];
-    for (int i = (int)threadIdx.x; i < 9 * kCH; i += 256) w[i] = W4[i];
-    __syncthreads();
-    const size_t total = (size_t)ncrops * kCropPix;
-    for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total;
-         g += (size_t)gridDim.x * 256) {
-        const int crop = (int)(g / kCropPix), p = (int)(g - (size_t)crop * kCropPix);
-        const int x = p % kCrop, y = p / kCrop;
-        float acc = 0.0f;
-        for (int t = 0; t < 9; ++t) {
-            const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-            if ((unsigned)yy >= (unsigned)kCrop || (unsigned)xx >= (unsigned)kCrop) continue;
-            const bf16_t* h = H + (((size_t)crop * kCrop + yy) * kCrop + xx) * kCH;
-            for (int c8 = 0; c8 < 4; ++c8) {
-                const u32x4 v = *reinterpret_cast<const u32x4*>(h + 8 * c8);
-                for (int k = 0; k < 4; ++k) {
-                    acc += w[t * kCH + 8 * c8 + 2 * k] * bf2f((bf16_t)(v[k] & 0xFFFFu));
-                    acc += w[t * kCH + 8 * c8 + 2 * k + 1] * bf2f((bf16_t)(v[k] >> 16));
+    __shared__ __align__(16) bf16_t hl[(kRowsPerItem + 2) * kHaloCols * PIXP];
+    const int tid = (int)threadIdx.x;
+    for (int i = tid; i < 9 * kCH; i += 256) w[i] = W4[i];
+    const int items = ncrops * kItemsPerCrop;
+    for (int item = (int)blockIdx.x; item < items; item += (int)gridDim.x) {
+        const int crop = item / kItemsPerCrop;
+        const int y0 = (item - crop * kItemsPerCrop) * kRowsPerItem;
+        __syncthreads();
+        for (int i = tid; i < (kRowsPerItem + 2) * kHaloCols * 4; i += 256) {
+            const int px = i >> 2, c8 = i & 3;
+            const int r = px / kHaloCols, cc = px - r * kHaloCols;
+            const int y = y0 - 1 + r, x = cc - 1;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if ((unsigned)y < (unsigned)kCrop && (unsigned)x < (unsigned)kCrop)
+                v = *reinterpret_cast<const u32x4*>(
+                    H + (((size_t)crop * kCrop + y) * kCrop + x) * kCH + 8 * c8);
+            *reinterpret_cast<u32x4*>(hl + px * PIXP + 8 * c8) = v;
+        }
+        __syncthreads();
+        for (int q = tid; q < kRowsPerItem * kCrop; q += 256) {
+            const int r = q / kCrop, x = q - r * kCrop;
+            float acc = 0.0f;
+            for (int t = 0; t < 9; ++t) {
+                const int ky = t / 3, kx = t - 3 * ky;
+                const bf16_t* h = hl + ((r + ky) * kHaloCols + x + kx) * PIXP;
+                for (int c8 = 0; c8 < 4; ++c8) {
+                    const u32x4 v = *reinterpret_cast<const u32x4*>(h + 8 * c8);
+                    for (int k = 0; k < 4; ++k) {
+                        acc += w[t * kCH + 8 * c8 + 2 * k] * bf2f((bf16_t)(v[k] & 0xFFFFu));
+                        acc += w[t * kCH + 8 * c8 + 2 * k + 1] * bf2f((bf16_t)(v[k] >> 16));
+                    }
                 }
             }
+            const size_t g = (size_t)crop * kCropPix + (size_t)(y0 + r) * kCrop + x;
+            const float d = acc + b4;
+            if (dphi) dphi[g] = d;
+            if (out) out[g] = phic[g] + d;
         }
-        const float d = acc + b4;
-        if (dphi) dphi[g] = d;
-        if (out) out[g] = phic[g] + d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused inference: all four layers of one crop in one workgroup
+// ---------------------------------------------------------------------------
+// The layer-by-layer kernels stream every 32-channel activation map through
+// HBM (96 x 96 x 32 bf16 = 590 KB per crop and layer). k_sur_infer keeps them
+// on chip: one workgroup walks a crop top to bottom two rows per step, each
+// layer trailing the one before by three rows, and holds the last six rows of
+// X, h1, h2 and of the output layer's per-tap partials in LDS rings. Every
+// stage of a step reads only rows finished in earlier steps and writes a ring
+// slot nobody reads in that step, so one barrier per step suffices.
+//
+//   step s: X rows   [2s+3, 2s+5)   written to the ring (fetched in step s-1)
+//           h1 rows  [2s,   2s+2)   conv1, waves 0-5
+//           h2 rows  [2s-3, 2s-1)   conv2, waves 0-5
+//           h3 rows  [2s-6, 2s-4)   conv3 + output partials, waves 6-11
+//           out rows [2s-9, 2s-7)   sum of partials, waves 0-2
+//
+// MFMA operands: A = weights (row m holds output channel perm(m), so that the
+// accumulator of lane (pixel, half h) holds channels 16h .. 16h+15 in order),
+// B = activations (lane = pixel). Each wave keeps its layer's A fragments in
+// registers for the whole kernel. The output layer (32 -> 1, 3x3) is an MFMA
+// on conv3's accumulators: P[t][p] = sum_c W4[t][c] h3[p][c] for the nine
+// taps t, with W4 split into bf16 hi + lo parts (16 significant bits), and
+// dphi[y][x] = b4 + sum_t P[t][(y, x) + off(t)] in tap order.
+// Rows outside the crop are written as zeros (the zero padding of every
+// layer's input); ring columns 0 and 97 stay zero.
+constexpr int kIT = 768;                       // 12 waves
+constexpr int kRing = 6;                       // rows per ring
+constexpr int kHP = kCH + 8;                   // h ring pixel pitch (bf16), 80 B
+constexpr int kRowH = kHaloCols * kHP;
+constexpr int kXP = 4;                         // X ring: 4 channels per pixel
+constexpr int kRowX = kHaloCols * kXP;
+constexpr int kRowP = 9 * kHaloCols;           // P ring row: [tap][column] fp32
+constexpr int kFirstStep = -3, kLastStep = 52;
+
+struct InferLDS {
+    bf16_t h1[kRing * kRowH];
+    bf16_t h2[kRing * kRowH];
+    bf16_t x[kRing * kRowX];
+    float p[kRing * kRowP];
+    float bias[3 * kCH];
+};
+
+__device__ __forceinline__ int ring(int row) { return (row + 16 * kRing) % kRing; }
+__device__ __forceinline__ bool in_crop(int row) { return (unsigned)row < (unsigned)kCrop; }
+// MFMA row m -> output channel, so that accumulator r of lane half h is
+// channel 16 h + r
+__device__ __forceinline__ int perm_row(int m) {
+    return 16 * ((m >> 2) & 1) + 4 * (m >> 3) + (m & 3);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// bias + ReLU + bf16 of a lane's 16 channels, packed as two 8-channel chunks
+__device__ __forceinline__ void relu_pack(const f32x16& acc, const float* bias, u32x4* lo,
+                                          u32x4* hi) {
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float a = fmaxf(acc[2 * k] + bias[2 * k], 0.0f);
+        const float b = fmaxf(acc[2 * k + 1] + bias[2 * k + 1], 0.0f);
+        w[k] = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+    }
+    *lo = u32x4{w[0], w[1], w[2], w[3]};
+    *hi = u32x4{w[4], w[5], w[6], w[7]};
+}
+
+// 32 -> 32 convolution of one 32-pixel tile of row y from ring `src`
+__device__ __forceinline__ f32x16 conv32_tile(const bf16_t* src, const bf16x8* wa, int y,
+                                              int tile, int col, int half) {
+    const bf16_t* base = src + (32 * tile + col) * kHP + 8 * half;
+    const int r0 = ring(y - 1) * kRowH, r1 = ring(y) * kRowH, r2 = ring(y + 1) * kRowH;
+    f32x16 acc = {};
+#pragma unroll
+    for (int s = 0; s < 18; ++s) {
+        const int t = s >> 1, ky = t / 3, kx = t - 3 * ky;
+        const int ro = ky == 0 ? r0 : ky == 1 ? r1 : r2;
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(base + ro + kx * kHP + 16 * (s & 1));
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// 4 -> 32 convolution (k = 4 tap + channel, 36 of 48 used) of one tile
+__device__ __forceinline__ f32x16 conv4_tile(const bf16_t* xr, const bf16x8* wa, int y, int tile,
+                                             int col, int half) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        uint32_t v[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int t = 4 * s + 2 * half + j;
+            if (t < 9) {
+                const int ky = t / 3, kx = t - 3 * ky;
+                const u32x2 q = *reinterpret_cast<const u32x2*>(
+                    xr + ring(y + ky - 1) * kRowX + (32 * tile + col + kx) * kXP);
+                v[2 * j] = q[0];
+                v[2 * j + 1] = q[1];
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], as_bf16x8(u32x4{v[0], v[1], v[2], v[3]}),
+                                                      acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void store_h(bf16_t* dst, int y, int tile, int col, int half, u32x4 lo,
+                                        u32x4 hi) {
+    bf16_t* o = dst + ring(y) * kRowH + (32 * tile + col + 1) * kHP + 16 * half;
+    *reinterpret_cast<u32x4*>(o) = lo;
+    *reinterpret_cast<u32x4*>(o + 8) = hi;
+}
+
+__global__ __launch_bounds__(kIT, 1) void k_sur_infer(
+    const bf16_t* X, const bf16_t* W1, const float* b1, const bf16_t* W2, const float* b2,
+    const bf16_t* W3, const float* b3, const float* W4, float b4, const float* phic, float* dphi,
+    float* pred, int ncrops) {
+    __shared__ InferLDS sm;
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    const int m = perm_row(col);
+
+    // zero every ring once: the halo columns are never written again
+    {
+        uint32_t* z = reinterpret_cast<uint32_t*>(&sm);
+        for (int i = tid; i < (int)(sizeof(InferLDS) / 4); i += kIT) z[i] = 0u;
+    }
+    __syncthreads();
+    for (int i = tid; i < kCH; i += kIT) {
+        sm.bias[i] = b1[i];
+        sm.bias[kCH + i] = b2[i];
+        sm.bias[2 * kCH + i] = b3[i];
+    }
+
+    // register-resident A fragments
+    const bool early = wave < 6;           // waves 0-5: conv1 + conv2; 6-11: conv3 + out
+    bf16x8 wa[18];
+    {
+        const bf16_t* wr = (early ? W2 : W3) + (size_t)m * (9 * kCH);
+#pragma unroll
+        for (int s = 0; s < 18; ++s)
+            wa[s] = *reinterpret_cast<const bf16x8*>(wr + 16 * s + 8 * half);
+    }
+    // wx: conv1 fragments (waves 0-5: k-steps 0-2) or the output layer's
+    // (waves 6-11: hi 0, lo 0, hi 1, lo 1); one array keeps both in one set of
+    // registers
+    bf16x8 wx[4];
+    if (early) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            uint32_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t pr = 0u;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int kk = 16 * s + 8 * half + 2 * k + e;
+                    const int t = kk >> 2, ci = kk & 3;
+                    const uint32_t b = t < 9 ? (uint32_t)W1[(size_t)m * (9 * kC0) + t * kC0 + ci]
+                                             : 0u;
+                    pr |= b << (16 * e);
+                }
+                v[k] = pr;
+            }
+            wx[s] = as_bf16x8(u32x4{v[0], v[1], v[2], v[3]});
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            uint32_t vh[4], vl[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t ph = 0u, pl = 0u;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int c = 16 * half + 8 * j + 2 * k + e;
+                    const float w = col < 9 ? W4[col * kCH + c] : 0.0f;
+                    const bf16_t hb = f2bf(w);
+                    const bf16_t lb = f2bf(w - bf2f(hb));
+                    ph |= (uint32_t)hb << (16 * e);
+                    pl |= (uint32_t)lb << (16 * e);
+                }
+                vh[k] = ph;
+                vl[k] = pl;
+            }
+            wx[2 * j] = as_bf16x8(u32x4{vh[0], vh[1], vh[2], vh[3]});
+            wx[2 * j + 1] = as_bf16x8(u32x4{vl[0], vl[1], vl[2], vl[3]});
+        }
+    }
+    __syncthreads();
+
+    // job coordinates within a step: waves 0-5 and 6-11 each cover 2 rows x 3 tiles
+    const int jw = early ? wave : wave - 6;
+    const int jrow = jw / 3, jtile = jw - 3 * jrow;
+    // X fetch (waves 3-5) and output (waves 0-2): one pixel per lane, 2 rows
+    const int io = early ? (wave < 3 ? tid : tid - 192) : 0;
+    const int iorow = io / kCrop, iox = io - iorow * kCrop;
+
+    for (int crop = (int)blockIdx.x; crop < ncrops; crop += (int)gridDim.x) {
+        const bf16_t* Xc = X + (size_t)crop * kCropPix * kC0;
+        const float* phicc = phic + (size_t)crop * kCropPix;
+        u32x2 xreg = {0u, 0u};      // X rows of the first step are outside the crop
+        float preg = 0.0f;
+        for (int s = kFirstStep; s <= kLastStep; ++s) {
+            if (early && wave >= 3) {
+                // X rows [2s+3, 2s+5): store what step s-1 fetched, fetch [2s+5, 2s+7)
+                const int yw = 2 * s + 3 + iorow;
+                *reinterpret_cast<u32x2*>(sm.x + ring(yw) * kRowX + (iox + 1) * kXP) = xreg;
+                const int yf = yw + 2;
+                xreg = in_crop(yf)
+                           ? *reinterpret_cast<const u32x2*>(Xc + ((size_t)yf * kCrop + iox) * kC0)
+                           : u32x2{0u, 0u};
+            }
+            float pnext = 0.0f;
+            if (early && wave < 3) {
+                const int yn = 2 * s - 7 + iorow;     // output row of step s+1
+                if (in_crop(yn)) pnext = phicc[yn * kCrop + iox];
+            }
+            if (early) {
+                // conv1: h1 row 2s + jrow
+                const int y1 = 2 * s + jrow;
+                u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
+                if (in_crop(y1)) {
+                    const f32x16 a = conv4_tile(sm.x, wx, y1, jtile, col, half);
+                    relu_pack(a, sm.bias + 16 * half, &lo, &hi);
+                }
+                store_h(sm.h1, y1, jtile, col, half, lo, hi);
+                // conv2: h2 row 2s - 3 + jrow
+                const int y2 = 2 * s - 3 + jrow;
+                lo = u32x4{0u, 0u, 0u, 0u};
+                hi = lo;
+                if (in_crop(y2)) {
+                    const f32x16 a = conv32_tile(sm.h1, wa, y2, jtile, col, half);
+                    relu_pack(a, sm.bias + kCH + 16 * half, &lo, &hi);
+                }
+                store_h(sm.h2, y2, jtile, col, half, lo, hi);
+                // output rows [2s-9, 2s-7): b4 + partials in tap order
+                if (wave < 3) {
+                    const int yo = 2 * s - 9 + iorow;
+                    if (in_crop(yo)) {
+                        float d = b4;
+#pragma unroll
+                        for (int t = 0; t < 9; ++t) {
+                            const int ky = t / 3, kx = t - 3 * ky;
+                            d += sm.p[ring(yo + ky - 1) * kRowP + t * kHaloCols + iox + kx];
+                        }
+                        const size_t g = (size_t)crop * kCropPix + (size_t)yo * kCrop + iox;
+                        if (dphi) dphi[g] = d;
+                        pred[g] = preg + d;
+                    }
+                }
+            } else {
+                // conv3 (h3 row 2s - 6 + jrow) and the output layer's partials
+                const int y3 = 2 * s - 6 + jrow;
+                f32x16 pacc = {};
+                if (in_crop(y3)) {
+                    const f32x16 a = conv32_tile(sm.h2, wa, y3, jtile, col, half);
+                    u32x4 lo, hi;
+                    relu_pack(a, sm.bias + 2 * kCH + 16 * half, &lo, &hi);
+                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[0], as_bf16x8(lo), pacc,
+                                                                   0, 0, 0);
+                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[1], as_bf16x8(lo), pacc,
+                                                                   0, 0, 0);
+                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[2], as_bf16x8(hi), pacc,
+                                                                   0, 0, 0);
+                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[3], as_bf16x8(hi), pacc,
+                                                                   0, 0, 0);
+                }
+                // lane (pixel, half 0) holds taps 0-3 and 8, half 1 taps 4-7
+                float* pr = sm.p + ring(y3) * kRowP + 32 * jtile + col + 1;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) pr[(4 * half + r) * kHaloCols] = pacc[r];
+                if (half == 0) pr[8 * kHaloCols] = pacc[4];
+            }
+            preg = pnext;
+            __syncthreads();
+        }
     }
 }
 
@@ -474,6 +775,88 @@ __global__ __launch_bounds__(256) void k_sur_writeback(float* phi, const pfh_cel
     }
 }
 
+// Acceptance test of every cell in one pass: the check field is read from
+// the check step's tile on the recorded crop window (as k_sur_recrop), err[n]
+// = relative L2 of the prediction against it (fp64 sums, lane tree, waves
+// ascending), and rejected[0] counts the cells with err > tol.
+__global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
+                                                   const pfh_cell_state* cells,
+                                                   const uint8_t* cls, const int* origin, int L,
+                                                   const float* pred, float tol, float* err,
+                                                   uint32_t* rejected) {
+    __shared__ double r1[4], r2[4];
+    const int n = (int)blockIdx.x;
+    const pfh_cell_state c = cells[n];
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    const float* tile = phi_chk + (size_t)n * pfh::kTileArea;
+    const int k = (int)cls[n];
+    double e = 0.0, s = 0.0;
+    for (int p = (int)threadIdx.x; p < kCropPix; p += 256) {
+        const int i = p % kCrop, j = p / kCrop;
+        int wx = ox + i;
+        if (wx >= L) wx -= L;
+        int wy = oy + j;
+        if (wy >= L) wy -= L;
+        const double b = (double)cell_phi(tile, c, k, L, wx, wy);
+        const double d = (double)pred[(size_t)n * kCropPix + p] - b;
+        e += d * d;
+        s += b * b;
+    }
+    e = pfh::wave_sum(e);
+    s = pfh::wave_sum(s);
+    if ((threadIdx.x & 63) == 0) {
+        r1[threadIdx.x >> 6] = e;
+        r2[threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double E = ((r1[0] + r1[1]) + r1[2]) + r1[3];
+        const double T = ((r2[0] + r2[1]) + r2[2]) + r2[3];
+        const float r = T > 0.0 ? (float)sqrt(E / T) : (E > 0.0 ? 1e30f : 0.0f);
+        err[n] = r;
+        if (!(r <= tol)) atomicAdd(rejected, 1u);
+    }
+}
+
+// Accepted block, one workgroup of 1024 threads per cell: writes the
+// prediction into the tile's active window (window pixels outside the crop
+// become 0), scatters q(phi) into the zeroed aggregate S and stores the
+// moments, visiting the window with the update's thread mapping so that V,
+// Cx, Cy equal k_init_moments' sums bit for bit (k_scatter_all /
+// k_init_moments semantics, src/kernels.cu:1098-1190).
+__global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_state* cells,
+                                                         const uint8_t* cls, const int* origin,
+                                                         const float* pred, int L, int P,
+                                                         uint32_t* S, uint32_t* flags) {
+    const int n = (int)blockIdx.x;
+    const int k = (int)cls[n];
+    const pfh::Shape sc = pfh::shape_of(k);
+    const int gx0 = cells[n].gx0, gy0 = cells[n].gy0;
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    float* t = phi + (size_t)n * pfh::kTileArea + (size_t)sc.ty0 * pfh::kTilePitch + sc.tx0;
+    const float* pr = pred + (size_t)n * kCropPix;
+    bool ovf = false, clampf = false;
+    const pfh::WindowMoments m = pfh::window_moments(k, [&](int a, int b) {
+        const int wx = pfh::wrapi(gx0 + a, L), wy = pfh::wrapi(gy0 + b, L);
+        int i = wx - ox;
+        if (i < 0) i += L;
+        int j = wy - oy;
+        if (j < 0) j += L;
+        const float v = (i < kCrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
+        t[(size_t)b * pfh::kTilePitch + a] = v;
+        const uint32_t q = pfh::q_of(v);
+        if (q) {
+            const uint32_t old = atomicAdd(&S[(size_t)wy * P + wx], q);
+            ovf |= old > 0xFFFFFFFFu - q;
+            clampf |= v * v > pfh::kQClampPhiSq;
+        }
+        return v;
+    });
+    if (ovf) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
+    if (clampf) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
+    if (threadIdx.x == 0) pfh::store_moments(&cells[n], m);
+}
+
 // The polarity of K skipped steps: the plain step's tumble draw
 // (src/kernels.cu:98-114) for steps step0 .. step0+K-1, in order.
 __global__ void k_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
@@ -552,9 +935,27 @@ int pfh_sur_conv(int cin, const uint16_t* X, const uint16_t* W, const float* bia
 int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic, float* dphi,
                 float* out, int ncrops, void* stream) {
     if (!H || !W4 || ncrops <= 0 || (out && !phic)) return invalid("sur_out");
-    hipLaunchKernelGGL(sur::k_sur_out, dim3(2048), dim3(256), 0, as_stream(stream), H, W4, b4,
-                       phic, dphi, out, ncrops);
+    hipLaunchKernelGGL(sur::k_sur_out, dim3((unsigned)sur_grid()), dim3(256), 0,
+                       as_stream(stream), H, W4, b4, phic, dphi, out, ncrops);
     return launch_status("k_sur_out");
+}
+
+int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
+                  const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
+                  const float* phic, float* dphi, float* pred, int ncrops, void* stream) {
+    if (!X || !W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !W4 || !phic || !pred || ncrops <= 0)
+        return invalid("sur_infer");
+    if (((uintptr_t)X | (uintptr_t)W2 | (uintptr_t)W3) & 15u) return invalid("sur_infer alignment");
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    const int grid = ncrops < cus ? ncrops : cus;
+    hipLaunchKernelGGL(sur::k_sur_infer, dim3((unsigned)grid), dim3(sur::kIT), 0,
+                       as_stream(stream), X, W1, b1, W2, b2, W3, b3, W4, b4, phic, dphi, pred,
+                       ncrops);
+    return launch_status("k_sur_infer");
 }
 
 int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
@@ -622,6 +1023,28 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
     hipLaunchKernelGGL(sur::k_sur_writeback, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
                        phi, cells, cls, origin, pred, accept, L);
     return launch_status("k_sur_writeback");
+}
+
+int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
+                  const int* origin, int N, int L, const float* pred, float tol, float* err,
+                  uint32_t* rejected, void* stream) {
+    if (!phi_chk || !cells || !cls || !origin || !pred || !err || !rejected || N <= 0 ||
+        L < sur::kCrop)
+        return invalid("sur_check");
+    hipLaunchKernelGGL(sur::k_sur_check, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
+                       phi_chk, cells, cls, origin, L, pred, tol, err, rejected);
+    return launch_status("k_sur_check");
+}
+
+int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,
+                   const float* pred, int N, int L, int P, uint32_t* S, uint32_t* flags,
+                   void* stream) {
+    if (!phi || !cells || !cls || !origin || !pred || !S || !flags || N <= 0 ||
+        L < sur::kCrop || P < L)
+        return invalid("sur_commit");
+    hipLaunchKernelGGL(sur::k_sur_commit, dim3((unsigned)N), dim3(pfh::kNT), 0,
+                       as_stream(stream), phi, cells, cls, origin, pred, L, P, S, flags);
+    return launch_status("k_sur_commit");
 }
 
 int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,

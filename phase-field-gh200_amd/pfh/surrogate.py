@@ -129,11 +129,13 @@ def bf16_bits_torch(x: torch.Tensor) -> torch.Tensor:
 class Workspace:
     """Activations and gradients for up to `n` crops."""
 
-    def __init__(self, n: int, device, train: bool = False):
+    def __init__(self, n: int, device, train: bool = False, hidden: bool = True):
         z = dict(device=device)
         self.n = n
         self.X0 = torch.zeros(n * PIX * C0, dtype=torch.int16, **z)
-        self.h = [torch.zeros(n * PIX * CH, dtype=torch.int16, **z) for _ in range(3)]
+        # hidden maps of the layer-by-layer kernels (not needed by infer())
+        self.h = [torch.zeros(n * PIX * CH, dtype=torch.int16, **z)
+                  for _ in range(3 if hidden or train else 0)]
         self.phic = torch.zeros(n * PIX, dtype=torch.float32, **z)
         self.dphi = torch.zeros(n * PIX, dtype=torch.float32, **z)
         self.pred = torch.zeros(n * PIX, dtype=torch.float32, **z)
@@ -160,6 +162,20 @@ def forward(net: Net, ws: Workspace, n: int, X0=None, phic=None, pred=True, stre
     abi.check(lib.pfh_sur_out(_ptr(ws.h[2]), _ptr(b["W4"]), net.b4(), _ptr(phic),
                               _ptr(ws.dphi), _ptr(ws.pred) if pred else None, n, stream),
               "out")
+
+
+def infer(net: Net, ws: Workspace, n: int, W4=None, b4=None, dphi=True, stream=None):
+    """The whole network in one kernel (pfh_sur_infer): ws.pred = ws.phic +
+    dphi for n crops, and ws.dphi when `dphi`. W4/b4 default to the raw
+    output layer (no scale)."""
+    W4 = net.views["W4"] if W4 is None else W4
+    b4 = net.b4() if b4 is None else b4
+    b = net.views
+    abi.check(net.lib.pfh_sur_infer(_ptr(ws.X0), _ptr(net.wb["W1"]), _ptr(b["b1"]),
+                                    _ptr(net.wb["W2"]), _ptr(b["b2"]), _ptr(net.wb["W3"]),
+                                    _ptr(b["b3"]), _ptr(W4), float(b4), _ptr(ws.phic),
+                                    _ptr(ws.dphi) if dphi else None, _ptr(ws.pred), n, stream),
+              "pfh_sur_infer")
 
 
 def backward(net: Net, ws: Workspace, n: int, target: torch.Tensor, X0=None, stream=None):
@@ -314,7 +330,7 @@ class SurrogateRunner:
         self.lib = abi.load()
         dev = sim.dev
         N = sim.N
-        self.ws = Workspace(N, dev)
+        self.ws = Workspace(N, dev, hidden=False)
         self.W4d, self.b4d = deploy_weights(net)
         self.phi_chk = torch.zeros_like(sim.phi[0])
         self.S_chk = torch.zeros(2 * sim.words, dtype=torch.int32, device=dev)
@@ -325,7 +341,7 @@ class SurrogateRunner:
         self.flags_chk = torch.zeros_like(sim.flags)
         self.chk = torch.zeros(N * PIX, dtype=torch.float32, device=dev)
         self.err = torch.zeros(N, dtype=torch.float32, device=dev)
-        self.accept = torch.ones(N, dtype=torch.int32, device=dev)
+        self.rejected = torch.zeros(1, dtype=torch.int32, device=dev)
         self.blocks = 0
         self.fallbacks = 0
 
@@ -355,16 +371,7 @@ class SurrogateRunner:
         sp = sim._sp()
         N = sim.N
         crop_sim(sim, self.ws)
-        b = self.net.views
-        abi.check(lib.pfh_sur_conv(8, _ptr(self.ws.X0), _ptr(self.net.wb["W1"]), _ptr(b["b1"]),
-                                   None, _ptr(self.ws.h[0]), N, 1, sp), "conv1")
-        abi.check(lib.pfh_sur_conv(32, _ptr(self.ws.h[0]), _ptr(self.net.wb["W2"]),
-                                   _ptr(b["b2"]), None, _ptr(self.ws.h[1]), N, 1, sp), "conv2")
-        abi.check(lib.pfh_sur_conv(32, _ptr(self.ws.h[1]), _ptr(self.net.wb["W3"]),
-                                   _ptr(b["b3"]), None, _ptr(self.ws.h[2]), N, 1, sp), "conv3")
-        abi.check(lib.pfh_sur_out(_ptr(self.ws.h[2]), _ptr(self.W4d), self.b4d,
-                                  _ptr(self.ws.phic), _ptr(self.ws.dphi), _ptr(self.ws.pred), N,
-                                  sp), "out")
+        infer(self.net, self.ws, N, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp)
         # check step: one plain step of K dt from phi_n on copies of the state
         self.cell_chk.copy_(sim.cell)
         self.cls_chk.copy_(sim.cls)
@@ -373,34 +380,31 @@ class SurrogateRunner:
         self.step_chk.fill_(s)
         a = self._check_args(s)
         abi.check(lib.pfh2d_launch_step(C.byref(a), sim.grid, sp, None, 0, 0.0, 0), "check step")
-        abi.check(lib.pfh_sur_recrop(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
-                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.chk), sp),
-                  "recrop check")
-        abi.check(lib.pfh_sur_rel_l2(_ptr(self.ws.pred), _ptr(self.chk), _ptr(self.err), N, sp),
-                  "rel_l2")
-        ok = bool((self.err <= self.tol).all().item())
+        self.rejected.zero_()
+        abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
+                                    _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
+                                    self.tol, _ptr(self.err), _ptr(self.rejected), sp),
+                  "check")
+        ok = int(self.rejected.item()) == 0
         self.blocks += 1
         if not ok:
             self.fallbacks += 1
             sim.run(K)
             return False
         pin = s % 2
-        self.accept.fill_(1)
-        abi.check(lib.pfh_sur_writeback(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
-                                        _ptr(self.ws.origin), _ptr(self.ws.pred),
-                                        _ptr(self.accept), N, sim.L, sp), "writeback")
         abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, s, K, sim.p.polarity_stream(),
                                      sim.p.p_tumble(), sp), "tumble")
         e = s + K
-        sim.S.zero_()
-        abi.check(lib.pfh2d_scatter_all(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
-                                        sim._s_ptr(e % 3), N, sim.L, sim.P, _ptr(sim.flags), sp),
-                  "scatter_all")
-        abi.check(lib.pfh2d_init_moments(_ptr(sim.cell), _ptr(sim.cls), _ptr(sim.phi[pin]), N,
-                                         sp), "init_moments")
+        # the next step reads S[e%3] and scatters into S[(e+1)%3]; it clears
+        # S[(e+2)%3] ahead by itself
+        sim.S_slot(e % 3).zero_()
+        sim.S_slot((e + 1) % 3).zero_()
+        abi.check(lib.pfh_sur_commit(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
+                                     _ptr(self.ws.origin), _ptr(self.ws.pred), N, sim.L, sim.P,
+                                     sim._s_ptr(e % 3), _ptr(sim.flags), sp), "commit")
         pin_e = e % 2
-        sim.stepc.copy_(torch.tensor([e if pin_e == 0 else e + 1, e if pin_e == 1 else e + 1],
-                                     dtype=torch.int64))
+        sim.stepc[pin_e].fill_(e)
+        sim.stepc[1 - pin_e].fill_(e + 1)
         sim.cursor.zero_()
         sim.steps_done = e
         return True

@@ -97,11 +97,13 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
     s0.record(sur.stream)
     for _ in range(blocks):
         runner.block()
-        errs_block.append(float(runner.err.max().item()))
+        errs_block.append(runner.err.max())
     s1.record(sur.stream)
     torch.cuda.synchronize()
+    errs_block = [float(e) for e in errs_block]
     ms_sur = s0.elapsed_time(s1)
     wall_sur = time.perf_counter() - t0
+    infer_t = time_infer(runner)
     ws = su.Workspace(plain.N, plain.dev)
     su.crop_sim(plain, ws)
     rel = _rel_l2_fields(sur, plain, ws.origin)
@@ -118,7 +120,34 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
         "rel_l2_vs_plain_max": float(rel.max()),
         "rel_l2_vs_plain_mean": float(rel.mean()),
         "fatal": bool(sur.fatal() or plain.fatal()),
+        "infer": infer_t,
     }
+
+
+# algorithmic MACs of one crop: conv1 (4 used input channels), conv2, conv3,
+# output layer
+INFER_FLOP_PER_CROP = 2 * su.PIX * 9 * (4 * su.CH + su.CH * su.CH * 2 + su.CH)
+
+
+def time_infer(runner: su.SurrogateRunner, reps: int = 20) -> dict:
+    """Average duration of the fused inference kernel on the runner's
+    stream (inputs of the last block) and its MFMA rate."""
+    sim = runner.sim
+    sp = sim._sp()
+    stream = sim.stream
+    ws = runner.ws
+    su.infer(runner.net, ws, sim.N, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        su.infer(runner.net, ws, sim.N, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    flop = INFER_FLOP_PER_CROP * sim.N
+    return {"kernel": "k_sur_infer", "us": us, "flop_per_launch": flop,
+            "tflops": flop / (us * 1e-6) / 1e12}
 
 
 def cmd_eval(a) -> int:

@@ -120,6 +120,55 @@ def test_forward_matches_oracle(R, c1_after):
     assert rel <= 1e-2
 
 
+def _w4_hi_lo(w4):
+    """The fused kernel's output-layer weights: bf16 hi + bf16 lo parts."""
+    from oracle import sur_ref as R
+    w = np.asarray(w4, np.float32)
+    hi = R.bf16_value(R.bf16_bits(w))
+    lo = R.bf16_value(R.bf16_bits((w - hi).astype(np.float32)))
+    return hi.astype(np.float64) + lo.astype(np.float64)
+
+
+def test_fused_inference_matches(R, c1_after):
+    """pfh_sur_infer (all layers in one kernel) against the layer-by-layer
+    kernels and the oracle; the crops are repeated 20 times (320 crops) so
+    that workgroups walk several crops, and every copy must agree bit for bit."""
+    sim = c1_after
+    n0, rep = sim.N, 20
+    n = n0 * rep
+    say(f"fused inference, {n} crops ({n0} distinct), random weights")
+    one = su.Workspace(n0, sim.dev)
+    su.crop_sim(sim, one)
+    ws = su.Workspace(n, sim.dev)
+    ws.X0.copy_(one.X0.repeat(rep))
+    ws.phic.copy_(one.phic.repeat(rep))
+    net = _random_net(11)
+    su.forward(net, ws, n)
+    d_layer = ws.dphi.clone()
+    ws.dphi.fill_(float("nan"))
+    ws.pred.fill_(float("nan"))
+    su.infer(net, ws, n)
+    torch.cuda.synchronize()
+    d = ws.dphi.view(rep, -1)
+    assert torch.isfinite(d).all()
+    assert torch.equal(d, d[:1].expand_as(d)), "copies of one crop differ"
+    assert torch.equal(ws.pred, ws.phic + ws.dphi)
+    df = d[0].double().cpu().numpy()
+    dl = d_layer.view(rep, -1)[0].double().cpu().numpy()
+    rel_layer = np.linalg.norm(df - dl) / np.linalg.norm(dl)
+    p = _oracle_params(R, net)
+    p.W4 = _w4_hi_lo(net.host()["W4"]).reshape(1, 9, 32)
+    X0 = R.bf16_value(_bits(one.X0, (n0, 96, 96, 8))).astype(np.float64)
+    ref = R.forward(X0, p)["dphi"].reshape(-1)
+    rel_oracle = np.linalg.norm(df - ref) / np.linalg.norm(ref)
+    say(f"dphi relative L2: {rel_layer:.2e} vs layer-by-layer kernels, {rel_oracle:.2e} vs "
+        f"oracle")
+    # hidden maps round to bf16 in both; rare one-ulp flips (fp32 sums in
+    # another order) propagate to the output
+    assert rel_layer <= 2e-3
+    assert rel_oracle <= 1e-2
+
+
 def test_backward_matches_oracle(R, c1_after):
     """Every gradient against the oracle's backward pass over the kernels'
     own forward activations."""
@@ -173,3 +222,39 @@ def test_training_lowers_loss():
     say(f"held-out increment error: network {e_net:.4e}, no-change {e_id:.4e}")
     assert hist[-1] < 0.7 * hist[0]
     assert e_net < 0.7 * e_id
+
+
+def test_accepted_block_commits_exactly():
+    """An accepted block (tolerance forced open) leaves the aggregate S and
+    the cell moments exactly as the plain path's own kernels rebuild them
+    from the written field (k_scatter_all / k_init_moments), and the check
+    error is finite."""
+    from pfh import abi
+    wl = params.workload_c1()
+    sim = sim2d.Sim2D(wl)
+    sim.step(20)
+    net = _random_net(3)
+    runner = su.SurrogateRunner(sim, net, K=8, tol=1e30)
+    say("one accepted block on C1, then rebuild S and moments")
+    assert runner.block()
+    torch.cuda.synchronize()
+    assert torch.isfinite(runner.err).all()
+    e = sim.steps_done
+    lib = abi.load()
+    S_ref = torch.zeros(sim.words, dtype=torch.int32, device=sim.dev)
+    flags = torch.zeros_like(sim.flags)
+    abi.check(lib.pfh2d_scatter_all(sim.phi[e % 2].data_ptr(), sim.cell.data_ptr(),
+                                    sim.cls.data_ptr(), S_ref.data_ptr(), sim.N, sim.L, sim.P,
+                                    flags.data_ptr(), None), "scatter_all")
+    cell_ref = sim.cell.clone()
+    abi.check(lib.pfh2d_init_moments(cell_ref.data_ptr(), sim.cls.data_ptr(),
+                                     sim.phi[e % 2].data_ptr(), sim.N, None), "init_moments")
+    torch.cuda.synchronize()
+    assert torch.equal(sim.S_slot(e % 3), S_ref)
+    assert torch.equal(sim.cell, cell_ref)
+    assert int(sim.S_slot(e % 3).count_nonzero()) > 0
+    # and the plain method continues from the committed state
+    sim.step(2)
+    sim.sync()
+    assert not sim.fatal()
+
