@@ -4,7 +4,7 @@
     python -m pfh.surrogate_train eval --weights weights.npz --blocks 25
 
 `train` generates crops from 2048^2 / 1024-cell monolayers (two samples per
-seed: 204,800 crops for seeds 1000..1099), trains with Adam and reports the
+seed, after 300 and 420 steps: 204,800 crops for seeds 1000..1099), trains with Adam and reports the
 held-out increment error of the network beside the no-change prediction.
 `eval` advances the 2048^2 / 1024-cell workload (seed 7) with the surrogate
 and, from the same state, with the plain method, and reports the relative
@@ -33,7 +33,7 @@ def log(msg: str) -> None:
 def cmd_train(a) -> int:
     seeds = list(range(a.seeds[0], a.seeds[1] + 1))
     t0 = time.perf_counter()
-    X, D = su.make_dataset(seeds, samples_per_seed=a.samples, warm=a.warm, log=log)
+    X, D = su.make_dataset(seeds, samples_per_seed=a.samples, warm=a.warm, gap=a.gap, log=log)
     n = D.numel() // su.PIX
     log(f"{n} crops in {time.perf_counter() - t0:.1f} s")
     held = min(2048, n // 10)
@@ -164,7 +164,8 @@ def main(argv=None) -> int:
     t.add_argument("--seeds", type=int, nargs=2, default=[1000, 1099])
     t.add_argument("--samples", type=int, default=2)
     t.add_argument("--warm", type=int, default=300)
-    t.add_argument("--epochs", type=int, default=3)
+    t.add_argument("--gap", type=int, default=120)
+    t.add_argument("--epochs", type=int, default=12)
     t.add_argument("--batch", type=int, default=128)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--scale", type=float, default=1e-3)

@@ -258,3 +258,21 @@ def test_accepted_block_commits_exactly():
     sim.sync()
     assert not sim.fatal()
 
+
+def test_committed_weights_accuracy_and_acceptance():
+    """The committed weights on the 2048^2 / 1024-cell monolayer (seed 7,
+    not a training seed): 10 surrogate blocks of 8 steps from step 300, every
+    block accepted, the field within 1e-3 relative L2 of the plain run on
+    every cell's crop window."""
+    from pfh import surrogate_train as st
+    assert su.WEIGHTS_PATH.exists(), su.WEIGHTS_PATH
+    net = su.Net.load(su.WEIGHTS_PATH)
+    say("committed weights: 10 blocks on C2 against the plain method")
+    r = st.evaluate(net, blocks=10)
+    say(f"accepted {r['accepted_blocks']}/{r['blocks']}, rel L2 vs plain max "
+        f"{r['rel_l2_vs_plain_max']:.3e} mean {r['rel_l2_vs_plain_mean']:.3e}, "
+        f"speedup {r['speedup']:.3f}")
+    assert not r["fatal"]
+    assert r["fallback_blocks"] == 0
+    assert r["rel_l2_vs_plain_max"] <= 1e-3
+
