@@ -475,9 +475,11 @@ int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8
 int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,
                    const float* pred, int N, int L, int P, uint32_t* S, uint32_t* flags,
                    void* stream);
-/* The polarity draws of K skipped plain steps, steps step0 .. step0+K-1
- * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. */
-int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+/* The polarity draws of K skipped plain steps, steps *step0 .. *step0+K-1
+ * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. step0 is a
+ * device pointer (the accepted-step counter), so the launch can be captured
+ * in a graph. */
+int pfh_sur_tumble(pfh_cell_state* cells, int N, const long long* step0, int K,
                    unsigned long long polarity_seed, double p_tumble, void* stream);
 
 #ifdef __cplusplus

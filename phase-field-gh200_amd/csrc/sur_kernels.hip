@@ -859,10 +859,11 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
 
 // The polarity of K skipped steps: the plain step's tumble draw
 // (src/kernels.cu:98-114) for steps step0 .. step0+K-1, in order.
-__global__ void k_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+__global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step0_dev, int K,
                              unsigned long long seed, double p_tumble) {
     const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (n >= N) return;
+    const unsigned long long step0 = (unsigned long long)*step0_dev;
     pfh_cell_state* c = cells + n;
     float theta = c->theta;
     uint32_t ctr = c->tumble_ctr;
@@ -1047,9 +1048,9 @@ int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const 
     return launch_status("k_sur_commit");
 }
 
-int pfh_sur_tumble(pfh_cell_state* cells, int N, unsigned long long step0, int K,
+int pfh_sur_tumble(pfh_cell_state* cells, int N, const long long* step0, int K,
                    unsigned long long polarity_seed, double p_tumble, void* stream) {
-    if (!cells || N <= 0 || K < 0) return invalid("sur_tumble");
+    if (!cells || !step0 || N <= 0 || K < 0) return invalid("sur_tumble");
     if (K == 0) return PFH_OK;
     hipLaunchKernelGGL(sur::k_sur_tumble, dim3((unsigned)((N + 127) / 128)), dim3(128), 0,
                        as_stream(stream), cells, N, step0, K, polarity_seed, p_tumble);

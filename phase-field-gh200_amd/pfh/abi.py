@@ -191,7 +191,7 @@ SIGNATURES = {
     "pfh_sur_adam": (_I, [_P, _P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P]),
     "pfh_sur_rel_l2": (_I, [_P, _P, _P, _I, _P]),
     "pfh_sur_writeback": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P]),
-    "pfh_sur_tumble": (_I, [_P, _I, C.c_ulonglong, _I, C.c_ulonglong, C.c_double, _P]),
+    "pfh_sur_tumble": (_I, [_P, _I, _P, _I, C.c_ulonglong, C.c_double, _P]),
 }
 
 # Entry points the ABI header declares; tests check the library exports all.

@@ -323,9 +323,18 @@ def deploy_weights(net: Net) -> tuple[torch.Tensor, float]:
 class SurrogateRunner:
     """Advances a Sim2D K steps at a time with the surrogate, checked per
     cell against one explicit step of size K dt, falling back to K plain
-    steps when any cell is rejected."""
+    steps when any cell is rejected.
 
-    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3):
+    A block is two halves: propose (crops, inference, check step, acceptance
+    test) and commit (polarity draws, exact S and moments, counters). Both
+    read the step index from the device counters, so with `graphs` each half
+    is captured once per residue of the step index mod 6 (the phi parity and
+    S rotation it touches) and replayed; the only host round trip per block
+    is the rejected-cell count."""
+
+    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3, graphs: bool = True):
+        if K <= 0 or K % 2:
+            raise ValueError("K must be a positive even number of steps")
         self.sim, self.net, self.K, self.tol = sim, net, K, tol
         self.lib = abi.load()
         dev = sim.dev
@@ -339,9 +348,11 @@ class SurrogateRunner:
         self.cur_chk = torch.zeros(2, dtype=torch.int64, device=dev)
         self.step_chk = torch.zeros(2, dtype=torch.int64, device=dev)
         self.flags_chk = torch.zeros_like(sim.flags)
-        self.chk = torch.zeros(N * PIX, dtype=torch.float32, device=dev)
         self.err = torch.zeros(N, dtype=torch.float32, device=dev)
         self.rejected = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.rejected_host = torch.zeros(1, dtype=torch.int32).pin_memory()
+        self.graphs = graphs
+        self._g = {}
         self.blocks = 0
         self.fallbacks = 0
 
@@ -364,20 +375,20 @@ class SurrogateRunner:
         a.physical_dt = sim.p.dt * self.K
         return a
 
-    def block(self) -> bool:
-        """Advance K steps; returns True when the surrogate was accepted."""
-        sim, lib, K = self.sim, self.lib, self.K
-        s = sim.steps_done
+    def _propose(self, s: int) -> None:
+        """Crops, inference, the check step (one plain step of K dt from
+        phi_n on copies of the state) and the acceptance test."""
+        sim, lib = self.sim, self.lib
         sp = sim._sp()
         N = sim.N
+        pin = s % 2
         crop_sim(sim, self.ws)
         infer(self.net, self.ws, N, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp)
-        # check step: one plain step of K dt from phi_n on copies of the state
         self.cell_chk.copy_(sim.cell)
         self.cls_chk.copy_(sim.cls)
         self.S_chk[:sim.words].zero_()
         self.cur_chk.zero_()
-        self.step_chk.fill_(s)
+        self.step_chk.copy_(sim.stepc[pin:pin + 1].expand(2))
         a = self._check_args(s)
         abi.check(lib.pfh2d_launch_step(C.byref(a), sim.grid, sp, None, 0, 0.0, 0), "check step")
         self.rejected.zero_()
@@ -385,16 +396,18 @@ class SurrogateRunner:
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
                                     self.tol, _ptr(self.err), _ptr(self.rejected), sp),
                   "check")
-        ok = int(self.rejected.item()) == 0
-        self.blocks += 1
-        if not ok:
-            self.fallbacks += 1
-            sim.run(K)
-            return False
+        self.rejected_host.copy_(self.rejected, non_blocking=True)
+
+    def _commit(self, s: int) -> None:
+        """Polarity draws of the K skipped steps, the prediction into the
+        tiles with an exact S and moments, counters advanced by K."""
+        sim, lib, K = self.sim, self.lib, self.K
+        sp = sim._sp()
+        N = sim.N
         pin = s % 2
-        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, s, K, sim.p.polarity_stream(),
-                                     sim.p.p_tumble(), sp), "tumble")
         e = s + K
+        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, _ptr(sim.stepc) + 8 * pin, K,
+                                     sim.p.polarity_stream(), sim.p.p_tumble(), sp), "tumble")
         # the next step reads S[e%3] and scatters into S[(e+1)%3]; it clears
         # S[(e+2)%3] ahead by itself
         sim.S_slot(e % 3).zero_()
@@ -402,11 +415,61 @@ class SurrogateRunner:
         abi.check(lib.pfh_sur_commit(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
                                      _ptr(self.ws.origin), _ptr(self.ws.pred), N, sim.L, sim.P,
                                      sim._s_ptr(e % 3), _ptr(sim.flags), sp), "commit")
-        pin_e = e % 2
-        sim.stepc[pin_e].fill_(e)
-        sim.stepc[1 - pin_e].fill_(e + 1)
+        # stepc[e % 2] = e (e % 2 == s % 2); the other entry is rewritten by
+        # the next plain step before anything reads it
+        sim.stepc.add_(K)
         sim.cursor.zero_()
-        sim.steps_done = e
+
+    def _captured(self, key: tuple) -> torch.cuda.CUDAGraph:
+        g = self._g.get(key)
+        if g is not None:
+            return g
+        sim = self.sim
+        g = torch.cuda.CUDAGraph()
+        st = torch.cuda.Stream(sim.dev)
+        st.wait_stream(sim.stream)
+        with torch.cuda.stream(st):
+            old = sim.stream
+            sim.stream = st
+            try:
+                with torch.cuda.graph(g, stream=st):
+                    (self._propose if key[0] == "propose" else self._commit)(key[1])
+            finally:
+                sim.stream = old
+        sim.stream.wait_stream(st)
+        torch.cuda.synchronize(sim.dev)
+        self._g[key] = g
+        return g
+
+    def prepare(self) -> None:
+        """Capture the graphs of the next three blocks' residues (capture
+        runs nothing), so that timed blocks replay only."""
+        if self.graphs:
+            for b in range(3):
+                r = (self.sim.steps_done + b * self.K) % 6
+                self._captured(("propose", r))
+                self._captured(("commit", r))
+
+    def block(self) -> bool:
+        """Advance K steps; returns True when the surrogate was accepted."""
+        sim, K = self.sim, self.K
+        s = sim.steps_done
+        if self.graphs:
+            self._captured(("propose", s % 6)).replay()
+        else:
+            self._propose(s)
+        torch.cuda.current_stream(sim.dev).synchronize()
+        ok = int(self.rejected_host[0]) == 0
+        self.blocks += 1
+        if not ok:
+            self.fallbacks += 1
+            sim.run(K)
+            return False
+        if self.graphs:
+            self._captured(("commit", s % 6)).replay()
+        else:
+            self._commit(s)
+        sim.steps_done = s + K
         return True
 
     def run(self, steps: int) -> None:

@@ -83,6 +83,7 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
     sur.capture_graph()
     sur.run(warm)
     runner = su.SurrogateRunner(sur, net, K=K, tol=tol)
+    runner.prepare()
     # timing: plain K-step blocks from graphs, surrogate blocks
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

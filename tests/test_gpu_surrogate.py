@@ -276,3 +276,34 @@ def test_committed_weights_accuracy_and_acceptance():
     assert r["fallback_blocks"] == 0
     assert r["rel_l2_vs_plain_max"] <= 1e-3
 
+
+def _state(sim):
+    return [sim.phi[0], sim.phi[1], sim.S, sim.cell, sim.cls, sim.stepc, sim.cursor, sim.flags]
+
+
+def test_graph_blocks_equal_eager_blocks():
+    """Captured propose/commit graphs, replayed over four blocks (all three
+    residues of the step mod 6, one of them twice) with eager work between
+    the replays, leave the state bit for bit equal to eager launches, for
+    accepted and for rejected (fallback) blocks."""
+    wl = params.workload_c1()
+    net = _random_net(3)
+    for tol in (1e30, 0.0):
+        sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
+        sa.step(10)
+        sb.step(10)
+        ra = su.SurrogateRunner(sa, net, tol=tol, graphs=False)
+        rb = su.SurrogateRunner(sb, net, tol=tol, graphs=True)
+        for b in range(4):
+            say(f"tol {tol:g}: block {b} from step {sa.steps_done}")
+            oka, okb = ra.block(), rb.block()
+            assert oka == okb == (tol > 1.0)
+            torch.cuda.synchronize()
+            assert sa.steps_done == sb.steps_done
+            for x, y in zip(_state(sa), _state(sb)):
+                assert torch.equal(x, y)
+            assert torch.equal(ra.err, rb.err)
+            # eager work between replays
+            sb.S_slot(0).sum()
+        assert not sa.fatal()
+
