@@ -127,6 +127,10 @@ typedef struct pfh2d_step_args {
     double p_tumble;
     int full_moment_every;
     unsigned long long clear_ahead_words;
+    /* NULL, or the launch does nothing unless *run_if != 0 (no reference
+     * counterpart: lets a captured surrogate block carry its plain-step
+     * fallback, gated on the device-side rejected-cell count). */
+    const int* run_if;
 } pfh2d_step_args;
 
 /* Kernel resource report: include/kernels.cuh:230-244 (KernelStats). */
@@ -440,6 +444,9 @@ int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic,
 int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
                   const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
                   const float* phic, float* dphi, float* pred, int ncrops, void* stream);
+/* y = bfloat16(x) with the conversion pfh_sur_infer applies to hidden
+ * activations (n even, y 4-byte aligned). */
+int pfh_sur_round_bf16(const float* x, uint16_t* y, int n, void* stream);
 /* Squared-error loss per crop (fp64) and g = scale (dphi - target). */
 int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
                  int ncrops, float scale, void* stream);
@@ -466,21 +473,31 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
                       const int* origin, const float* pred, const int* accept, int N, int L,
                       void* stream);
 /* Acceptance test: err[n] = relative L2 of pred against the check step's
- * field on the recorded crop windows; rejected[0] += cells with err > tol. */
+ * field on the recorded crop windows; rejected[0] += cells with err > tol;
+ * err_max (may be NULL) = running maximum of err as float bits. */
 int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
                   const int* origin, int N, int L, const float* pred, float tol, float* err,
-                  uint32_t* rejected, void* stream);
+                  uint32_t* rejected, uint32_t* err_max, void* stream);
 /* Accepted block: prediction into every cell's window, q(phi) scattered into
- * the zeroed aggregate S, moments as pfh2d_init_moments computes them. */
+ * the zeroed aggregate S, moments as pfh2d_init_moments computes them.
+ * Does nothing when skip_if is non-NULL and *skip_if != 0. */
 int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,
                    const float* pred, int N, int L, int P, uint32_t* S, uint32_t* flags,
-                   void* stream);
+                   const int* skip_if, void* stream);
+/* a[0..words) = b[0..words) = 0 unless *skip_if != 0 (16-byte aligned). */
+int pfh_sur_zero2(uint32_t* a, uint32_t* b, unsigned long long words, const int* skip_if,
+                  void* stream);
+/* End of a block: when *rejected == 0 the step counters advance by K and the
+ * queue cursors restart; otherwise *rejected_blocks (may be NULL) += 1. */
+int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, const int* rejected,
+                    uint32_t* rejected_blocks, void* stream);
 /* The polarity draws of K skipped plain steps, steps *step0 .. *step0+K-1
  * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. step0 is a
  * device pointer (the accepted-step counter), so the launch can be captured
- * in a graph. */
+ * in a graph. Does nothing when skip_if is non-NULL and *skip_if != 0. */
 int pfh_sur_tumble(pfh_cell_state* cells, int N, const long long* step0, int K,
-                   unsigned long long polarity_seed, double p_tumble, void* stream);
+                   unsigned long long polarity_seed, double p_tumble, const int* skip_if,
+                   void* stream);
 
 #ifdef __cplusplus
 }

@@ -861,6 +861,7 @@ __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
 __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
     __shared__ Smem sm;
     const int tid = (int)threadIdx.x;
+    if (A.run_if != nullptr && *A.run_if == 0) return;   // gated-off launch: no effect
 
     // Clear S[(step+2)%3] ahead: this slot is neither read nor scattered
     // into by this launch. Non-temporal stores keep it out of the caches.

@@ -44,6 +44,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // binary32 -> bfloat16, round to nearest even (NaN stays NaN)
 __host__ __device__ inline bf16_t f2bf(float f) {
@@ -54,6 +56,13 @@ __host__ __device__ inline bf16_t f2bf(float f) {
 }
 __host__ __device__ inline float bf2f(bf16_t h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// Two binary32 values -> packed bfloat16 (low half = a) with gfx950's
+// conversion instruction, round to nearest even (the same values as f2bf).
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 // ---------------------------------------------------------------------------
@@ -240,6 +249,13 @@ __global__ __launch_bounds__(256) void k_sur_conv(const bf16_t* X, const bf16_t*
     }
 }
 
+// The conversion k_sur_infer applies to hidden activations, exported so that
+// tests can pin its rounding against round-to-nearest-even.
+__global__ void k_sur_round_bf16(const float* x, uint32_t* y, int npairs) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < npairs) y[i] = pack_bf16(x[2 * i], x[2 * i + 1]);
+}
+
 // ---------------------------------------------------------------------------
 // output layer (32 -> 1), loss and their gradients (VALU, fp32)
 // ---------------------------------------------------------------------------
@@ -351,11 +367,9 @@ __device__ __forceinline__ void relu_pack(const f32x16& acc, const float* bias, 
                                           u32x4* hi) {
     uint32_t w[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float a = fmaxf(acc[2 * k] + bias[2 * k], 0.0f);
-        const float b = fmaxf(acc[2 * k + 1] + bias[2 * k + 1], 0.0f);
-        w[k] = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
-    }
+    for (int k = 0; k < 8; ++k)
+        w[k] = pack_bf16(fmaxf(acc[2 * k] + bias[2 * k], 0.0f),
+                         fmaxf(acc[2 * k + 1] + bias[2 * k + 1], 0.0f));
     *lo = u32x4{w[0], w[1], w[2], w[3]};
     *hi = u32x4{w[4], w[5], w[6], w[7]};
 }
@@ -413,7 +427,9 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(
     float* pred, int ncrops) {
     __shared__ InferLDS sm;
     const int tid = (int)threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    // wave-uniform: keeps the step's row and ring arithmetic on the scalar unit
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int half = lane >> 5, col = lane & 31;
     const int m = perm_row(col);
 
@@ -484,6 +500,9 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(
             wx[2 * j + 1] = as_bf16x8(u32x4{vl[0], vl[1], vl[2], vl[3]});
         }
     }
+    // Retire the fragment loads here: otherwise the counter-based waits in
+    // the step loop also wait for that step's X / phi prefetches.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     // job coordinates within a step: waves 0-5 and 6-11 each cover 2 rows x 3 tiles
@@ -783,7 +802,7 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
                                                    const pfh_cell_state* cells,
                                                    const uint8_t* cls, const int* origin, int L,
                                                    const float* pred, float tol, float* err,
-                                                   uint32_t* rejected) {
+                                                   uint32_t* rejected, uint32_t* err_max) {
     __shared__ double r1[4], r2[4];
     const int n = (int)blockIdx.x;
     const pfh_cell_state c = cells[n];
@@ -815,6 +834,7 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
         const float r = T > 0.0 ? (float)sqrt(E / T) : (E > 0.0 ? 1e30f : 0.0f);
         err[n] = r;
         if (!(r <= tol)) atomicAdd(rejected, 1u);
+        if (err_max) atomicMax(err_max, __float_as_uint(r));   // r >= 0: bits order as values
     }
 }
 
@@ -827,7 +847,9 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
 __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_state* cells,
                                                          const uint8_t* cls, const int* origin,
                                                          const float* pred, int L, int P,
-                                                         uint32_t* S, uint32_t* flags) {
+                                                         uint32_t* S, uint32_t* flags,
+                                                         const int* skip_if) {
+    if (skip_if && *skip_if) return;
     const int n = (int)blockIdx.x;
     const int k = (int)cls[n];
     const pfh::Shape sc = pfh::shape_of(k);
@@ -860,7 +882,8 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
 // The polarity of K skipped steps: the plain step's tumble draw
 // (src/kernels.cu:98-114) for steps step0 .. step0+K-1, in order.
 __global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step0_dev, int K,
-                             unsigned long long seed, double p_tumble) {
+                             unsigned long long seed, double p_tumble, const int* skip_if) {
+    if (skip_if && *skip_if) return;
     const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (n >= N) return;
     const unsigned long long step0 = (unsigned long long)*step0_dev;
@@ -879,6 +902,39 @@ __global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step
     }
     c->theta = theta;
     c->tumble_ctr = ctr;
+}
+
+// Two aggregate slots cleared for an accepted block (nothing when *skip_if).
+__global__ void k_sur_zero2(uint32_t* a, uint32_t* b, size_t words, const int* skip_if) {
+    if (skip_if && *skip_if) return;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t nvec = words >> 2;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+        reinterpret_cast<u32x4*>(a)[i] = z;
+        reinterpret_cast<u32x4*>(b)[i] = z;
+    }
+    for (size_t i = (nvec << 2) + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += stride) {
+        a[i] = 0u;
+        b[i] = 0u;
+    }
+}
+
+// End of a block: accepted -> the step counters advance by K and the queue
+// cursors restart; rejected -> the gated plain steps that follow advance
+// them, and rejected_blocks counts the block.
+__global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int K,
+                              const int* rejected, uint32_t* rejected_blocks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (*rejected) {
+        if (rejected_blocks) *rejected_blocks += 1u;
+        return;
+    }
+    stepc[0] += K;
+    stepc[1] += K;
+    cursor[0] = 0ull;
+    cursor[1] = 0ull;
 }
 
 }  // namespace sur
@@ -959,6 +1015,14 @@ int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const 
     return launch_status("k_sur_infer");
 }
 
+int pfh_sur_round_bf16(const float* x, uint16_t* y, int n, void* stream) {
+    if (!x || !y || n <= 0 || (n & 1) || (((uintptr_t)y) & 3u)) return invalid("sur_round_bf16");
+    const int np = n / 2;
+    hipLaunchKernelGGL(sur::k_sur_round_bf16, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), x, reinterpret_cast<uint32_t*>(y), np);
+    return launch_status("k_sur_round_bf16");
+}
+
 int pfh_sur_loss(const float* dphi, const float* target, float* g, double* crop_loss,
                  int ncrops, float scale, void* stream) {
     if (!dphi || !target || !g || !crop_loss || ncrops <= 0) return invalid("sur_loss");
@@ -1028,33 +1092,57 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
 
 int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
                   const int* origin, int N, int L, const float* pred, float tol, float* err,
-                  uint32_t* rejected, void* stream) {
+                  uint32_t* rejected, uint32_t* err_max, void* stream) {
     if (!phi_chk || !cells || !cls || !origin || !pred || !err || !rejected || N <= 0 ||
         L < sur::kCrop)
         return invalid("sur_check");
     hipLaunchKernelGGL(sur::k_sur_check, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
-                       phi_chk, cells, cls, origin, L, pred, tol, err, rejected);
+                       phi_chk, cells, cls, origin, L, pred, tol, err, rejected, err_max);
     return launch_status("k_sur_check");
 }
 
 int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,
                    const float* pred, int N, int L, int P, uint32_t* S, uint32_t* flags,
-                   void* stream) {
+                   const int* skip_if, void* stream) {
     if (!phi || !cells || !cls || !origin || !pred || !S || !flags || N <= 0 ||
         L < sur::kCrop || P < L)
         return invalid("sur_commit");
     hipLaunchKernelGGL(sur::k_sur_commit, dim3((unsigned)N), dim3(pfh::kNT), 0,
-                       as_stream(stream), phi, cells, cls, origin, pred, L, P, S, flags);
+                       as_stream(stream), phi, cells, cls, origin, pred, L, P, S, flags,
+                       skip_if);
     return launch_status("k_sur_commit");
 }
 
 int pfh_sur_tumble(pfh_cell_state* cells, int N, const long long* step0, int K,
-                   unsigned long long polarity_seed, double p_tumble, void* stream) {
+                   unsigned long long polarity_seed, double p_tumble, const int* skip_if,
+                   void* stream) {
     if (!cells || !step0 || N <= 0 || K < 0) return invalid("sur_tumble");
     if (K == 0) return PFH_OK;
     hipLaunchKernelGGL(sur::k_sur_tumble, dim3((unsigned)((N + 127) / 128)), dim3(128), 0,
-                       as_stream(stream), cells, N, step0, K, polarity_seed, p_tumble);
+                       as_stream(stream), cells, N, step0, K, polarity_seed, p_tumble, skip_if);
     return launch_status("k_sur_tumble");
+}
+
+int pfh_sur_zero2(uint32_t* a, uint32_t* b, unsigned long long words, const int* skip_if,
+                  void* stream) {
+    if (!a || !b || (((uintptr_t)a | (uintptr_t)b) & 15u)) return invalid("sur_zero2");
+    if (words == 0ull) return PFH_OK;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    hipLaunchKernelGGL(sur::k_sur_zero2, dim3((unsigned)(4 * cus)), dim3(256), 0,
+                       as_stream(stream), a, b, (size_t)words, skip_if);
+    return launch_status("k_sur_zero2");
+}
+
+int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, const int* rejected,
+                    uint32_t* rejected_blocks, void* stream) {
+    if (!stepc || !cursor || !rejected || K <= 0) return invalid("sur_advance");
+    hipLaunchKernelGGL(sur::k_sur_advance, dim3(1), dim3(64), 0, as_stream(stream), stepc, cursor,
+                       K, rejected, rejected_blocks);
+    return launch_status("k_sur_advance");
 }
 
 }  // extern "C"

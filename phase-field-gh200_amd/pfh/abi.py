@@ -68,6 +68,7 @@ class StepArgs(C.Structure):
         ("seed", C.c_ulonglong), ("polarity_seed", C.c_ulonglong),
         ("p_tumble", C.c_double), ("full_moment_every", C.c_int),
         ("clear_ahead_words", C.c_ulonglong),
+        ("run_if", C.c_void_p),
     ]
 
 
@@ -78,7 +79,7 @@ class KernelStats(C.Structure):
                 ("warps_per_sm", C.c_int), ("occupancy", C.c_double)]
 
 
-assert C.sizeof(StepArgs) == 216, C.sizeof(StepArgs)
+assert C.sizeof(StepArgs) == 224, C.sizeof(StepArgs)
 
 # ---- 3D records (include/pf3d/kernels.cuh) ----
 FLAG3_COUNT = 10
@@ -180,8 +181,11 @@ SIGNATURES = {
     "pfh_sur_recrop": (_I, [_P, _P, _P, _P, _I, _I, _P, _P]),
     "pfh_sur_conv": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _P]),
     "pfh_sur_out": (_I, [_P, _P, _F, _P, _P, _P, _I, _P]),
-    "pfh_sur_check": (_I, [_P, _P, _P, _P, _I, _I, _P, _F, _P, _P, _P]),
-    "pfh_sur_commit": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
+    "pfh_sur_check": (_I, [_P, _P, _P, _P, _I, _I, _P, _F, _P, _P, _P, _P]),
+    "pfh_sur_commit": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    "pfh_sur_zero2": (_I, [_P, _P, C.c_ulonglong, _P, _P]),
+    "pfh_sur_advance": (_I, [_P, _P, _I, _P, _P, _P]),
+    "pfh_sur_round_bf16": (_I, [_P, _P, _I, _P]),
     "pfh_sur_infer": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _I, _P]),
     "pfh_sur_loss": (_I, [_P, _P, _P, _P, _I, _F, _P]),
     "pfh_sur_out_bwd": (_I, [_P, _P, _P, _P, _P, _I, _P]),
@@ -191,7 +195,7 @@ SIGNATURES = {
     "pfh_sur_adam": (_I, [_P, _P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P]),
     "pfh_sur_rel_l2": (_I, [_P, _P, _P, _I, _P]),
     "pfh_sur_writeback": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P]),
-    "pfh_sur_tumble": (_I, [_P, _I, _P, _I, C.c_ulonglong, C.c_double, _P]),
+    "pfh_sur_tumble": (_I, [_P, _I, _P, _I, C.c_ulonglong, C.c_double, _P, _P]),
 }
 
 # Entry points the ABI header declares; tests check the library exports all.

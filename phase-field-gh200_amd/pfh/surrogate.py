@@ -323,14 +323,17 @@ def deploy_weights(net: Net) -> tuple[torch.Tensor, float]:
 class SurrogateRunner:
     """Advances a Sim2D K steps at a time with the surrogate, checked per
     cell against one explicit step of size K dt, falling back to K plain
-    steps when any cell is rejected.
+    steps when any cell is rejected (whole-population fallback).
 
-    A block is two halves: propose (crops, inference, check step, acceptance
-    test) and commit (polarity draws, exact S and moments, counters). Both
-    read the step index from the device counters, so with `graphs` each half
-    is captured once per residue of the step index mod 6 (the phi parity and
-    S rotation it touches) and replayed; the only host round trip per block
-    is the rejected-cell count."""
+    A block is issued as one stream of launches with no host round trip:
+    propose (crops, inference, check step, acceptance test into a device
+    rejected-cell count), then the commit launches (polarity draws, exact S
+    and moments, counters advanced by K), which do nothing when the count is
+    non-zero, then K plain steps gated on the same count (`run_if`), which do
+    nothing when it is zero. Either way the state advances K steps, so the
+    next block is issued at once. With `graphs` the launches of a block are
+    captured once per residue of the step index mod 6 (the phi parity and S
+    rotation it touches) and replayed. stats() reads the outcome counters."""
 
     def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3, graphs: bool = True):
         if K <= 0 or K % 2:
@@ -350,11 +353,12 @@ class SurrogateRunner:
         self.flags_chk = torch.zeros_like(sim.flags)
         self.err = torch.zeros(N, dtype=torch.float32, device=dev)
         self.rejected = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.rejected_host = torch.zeros(1, dtype=torch.int32).pin_memory()
+        # [0] largest check error so far (float bits), [1] rejected blocks
+        self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
         self.graphs = graphs
         self._g = {}
+        self._fb = [self._fallback_args(k) for k in range(6)]
         self.blocks = 0
-        self.fallbacks = 0
 
     def _check_args(self, slot: int) -> abi.StepArgs:
         sim = self.sim
@@ -375,13 +379,24 @@ class SurrogateRunner:
         a.physical_dt = sim.p.dt * self.K
         return a
 
-    def _propose(self, s: int) -> None:
-        """Crops, inference, the check step (one plain step of K dt from
-        phi_n on copies of the state) and the acceptance test."""
-        sim, lib = self.sim, self.lib
+    def _fallback_args(self, slot: int) -> abi.StepArgs:
+        """The plain step of `slot`, gated on the rejected-cell count."""
+        sim = self.sim
+        a = abi.StepArgs()
+        C.memmove(C.addressof(a), C.addressof(sim._args[slot]), C.sizeof(a))
+        a.run_if = _ptr(self.rejected)
+        return a
+
+    def _issue(self, s: int) -> None:
+        """All launches of the block that starts at step s."""
+        sim, lib, K = self.sim, self.lib, self.K
         sp = sim._sp()
         N = sim.N
         pin = s % 2
+        e = s + K
+        rej = _ptr(self.rejected)
+        # propose: crops, inference, one plain step of K dt from phi_n on
+        # copies of the state, acceptance test
         crop_sim(sim, self.ws)
         infer(self.net, self.ws, N, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp)
         self.cell_chk.copy_(sim.cell)
@@ -394,34 +409,27 @@ class SurrogateRunner:
         self.rejected.zero_()
         abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
-                                    self.tol, _ptr(self.err), _ptr(self.rejected), sp),
+                                    self.tol, _ptr(self.err), rej, _ptr(self.counters), sp),
                   "check")
-        self.rejected_host.copy_(self.rejected, non_blocking=True)
-
-    def _commit(self, s: int) -> None:
-        """Polarity draws of the K skipped steps, the prediction into the
-        tiles with an exact S and moments, counters advanced by K."""
-        sim, lib, K = self.sim, self.lib, self.K
-        sp = sim._sp()
-        N = sim.N
-        pin = s % 2
-        e = s + K
+        # commit (no effect when rejected): the next step reads S[e%3] and
+        # scatters into S[(e+1)%3]; it clears S[(e+2)%3] ahead by itself
+        abi.check(lib.pfh_sur_zero2(sim._s_ptr(e % 3), sim._s_ptr((e + 1) % 3), sim.words, rej,
+                                    sp), "zero2")
         abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, _ptr(sim.stepc) + 8 * pin, K,
-                                     sim.p.polarity_stream(), sim.p.p_tumble(), sp), "tumble")
-        # the next step reads S[e%3] and scatters into S[(e+1)%3]; it clears
-        # S[(e+2)%3] ahead by itself
-        sim.S_slot(e % 3).zero_()
-        sim.S_slot((e + 1) % 3).zero_()
+                                     sim.p.polarity_stream(), sim.p.p_tumble(), rej, sp),
+                  "tumble")
         abi.check(lib.pfh_sur_commit(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
                                      _ptr(self.ws.origin), _ptr(self.ws.pred), N, sim.L, sim.P,
-                                     sim._s_ptr(e % 3), _ptr(sim.flags), sp), "commit")
-        # stepc[e % 2] = e (e % 2 == s % 2); the other entry is rewritten by
-        # the next plain step before anything reads it
-        sim.stepc.add_(K)
-        sim.cursor.zero_()
+                                     sim._s_ptr(e % 3), _ptr(sim.flags), rej, sp), "commit")
+        abi.check(lib.pfh_sur_advance(_ptr(sim.stepc), _ptr(sim.cursor), K, rej,
+                                      _ptr(self.counters) + 4, sp), "advance")
+        # fallback (no effect when accepted): K plain steps
+        for k in range(K):
+            abi.check(lib.pfh2d_launch_step(C.byref(self._fb[(s + k) % 6]), sim.grid, sp, None,
+                                            0, 0.0, 0), "fallback step")
 
-    def _captured(self, key: tuple) -> torch.cuda.CUDAGraph:
-        g = self._g.get(key)
+    def _captured(self, r: int) -> torch.cuda.CUDAGraph:
+        g = self._g.get(r)
         if g is not None:
             return g
         sim = self.sim
@@ -433,12 +441,12 @@ class SurrogateRunner:
             sim.stream = st
             try:
                 with torch.cuda.graph(g, stream=st):
-                    (self._propose if key[0] == "propose" else self._commit)(key[1])
+                    self._issue(r)
             finally:
                 sim.stream = old
         sim.stream.wait_stream(st)
         torch.cuda.synchronize(sim.dev)
-        self._g[key] = g
+        self._g[r] = g
         return g
 
     def prepare(self) -> None:
@@ -446,31 +454,26 @@ class SurrogateRunner:
         runs nothing), so that timed blocks replay only."""
         if self.graphs:
             for b in range(3):
-                r = (self.sim.steps_done + b * self.K) % 6
-                self._captured(("propose", r))
-                self._captured(("commit", r))
+                self._captured((self.sim.steps_done + b * self.K) % 6)
 
-    def block(self) -> bool:
-        """Advance K steps; returns True when the surrogate was accepted."""
-        sim, K = self.sim, self.K
+    def block(self) -> None:
+        """Issue the next K steps (surrogate, or plain when rejected)."""
+        sim = self.sim
         s = sim.steps_done
         if self.graphs:
-            self._captured(("propose", s % 6)).replay()
+            self._captured(s % 6).replay()
         else:
-            self._propose(s)
-        torch.cuda.current_stream(sim.dev).synchronize()
-        ok = int(self.rejected_host[0]) == 0
+            self._issue(s)
+        sim.steps_done = s + self.K
         self.blocks += 1
-        if not ok:
-            self.fallbacks += 1
-            sim.run(K)
-            return False
-        if self.graphs:
-            self._captured(("commit", s % 6)).replay()
-        else:
-            self._commit(s)
-        sim.steps_done = s + K
-        return True
+
+    def stats(self) -> dict:
+        """Blocks issued, rejected (fallback) blocks and the largest
+        per-cell check error so far (waits for the stream)."""
+        c = self.counters.cpu().numpy()
+        return {"blocks": self.blocks, "fallback_blocks": int(c[1]),
+                "accepted_blocks": self.blocks - int(c[1]),
+                "max_check_error": float(c[:1].view(np.float32)[0])}
 
     def run(self, steps: int) -> None:
         assert steps % self.K == 0

@@ -92,16 +92,14 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
     e1.record(plain.stream)
     torch.cuda.synchronize()
     ms_plain = e0.elapsed_time(e1)
-    errs_block = []
     t0 = time.perf_counter()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s0.record(sur.stream)
     for _ in range(blocks):
         runner.block()
-        errs_block.append(runner.err.max())
     s1.record(sur.stream)
     torch.cuda.synchronize()
-    errs_block = [float(e) for e in errs_block]
+    st = runner.stats()
     ms_sur = s0.elapsed_time(s1)
     wall_sur = time.perf_counter() - t0
     infer_t = time_infer(runner)
@@ -115,9 +113,9 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
         "surrogate_cell_steps_per_s": N * blocks * K / (ms_sur / 1e3),
         "speedup": ms_plain / ms_sur,
         "surrogate_wall_s": wall_sur,
-        "accepted_blocks": runner.blocks - runner.fallbacks,
-        "fallback_blocks": runner.fallbacks,
-        "max_check_error": max(errs_block) if errs_block else None,
+        "accepted_blocks": st["accepted_blocks"],
+        "fallback_blocks": st["fallback_blocks"],
+        "max_check_error": st["max_check_error"],
         "rel_l2_vs_plain_max": float(rel.max()),
         "rel_l2_vs_plain_mean": float(rel.mean()),
         "fatal": bool(sur.fatal() or plain.fatal()),

@@ -129,6 +129,36 @@ def _w4_hi_lo(w4):
     return hi.astype(np.float64) + lo.astype(np.float64)
 
 
+def test_hardware_bf16_rounding(R):
+    """The conversion the fused kernel applies to hidden activations rounds
+    to nearest even like the oracle, on ties (both parities), values just
+    off ties, subnormals, the largest finite values, infinities and signed
+    zeros; NaN stays NaN."""
+    from pfh import abi
+    rng = np.random.default_rng(4)
+    hi = rng.integers(0, 1 << 16, 4096, dtype=np.uint32) << 16
+    special = []
+    for h in (0x3F80, 0x3F81, 0x0001, 0x0080, 0x7F7F, 0xBF80, 0x8001):
+        for lo in (0x0000, 0x7FFF, 0x8000, 0x8001, 0xFFFF):
+            special.append((h << 16) | lo)
+    special += [0x00000001, 0x007FFFFF, 0x80000001, 0x7F7FFFFF, 0x7F800000, 0xFF800000,
+                0x00000000, 0x80000000, 0x7FC00000, 0x7F800001]
+    bits = np.concatenate([hi | rng.integers(0, 1 << 16, 4096, dtype=np.uint32),
+                           np.array(special, np.uint32)])
+    if bits.size % 2:
+        bits = np.append(bits, np.uint32(0))
+    x = bits.view(np.float32)
+    xt = torch.from_numpy(x.copy()).cuda()
+    yt = torch.zeros(x.size, dtype=torch.int16, device="cuda")
+    abi.check(abi.load().pfh_sur_round_bf16(xt.data_ptr(), yt.data_ptr(), x.size, None),
+              "round_bf16")
+    got = yt.cpu().numpy().view(np.uint16)
+    want = R.bf16_bits(x)
+    nan = np.isnan(x)
+    assert np.array_equal(got[~nan], want[~nan])
+    assert np.all(np.isnan(R.bf16_value(got[nan])))
+
+
 def test_fused_inference_matches(R, c1_after):
     """pfh_sur_infer (all layers in one kernel) against the layer-by-layer
     kernels and the oracle; the crops are repeated 20 times (320 crops) so
@@ -236,8 +266,9 @@ def test_accepted_block_commits_exactly():
     net = _random_net(3)
     runner = su.SurrogateRunner(sim, net, K=8, tol=1e30)
     say("one accepted block on C1, then rebuild S and moments")
-    assert runner.block()
-    torch.cuda.synchronize()
+    runner.block()
+    st = runner.stats()
+    assert st["fallback_blocks"] == 0 and st["accepted_blocks"] == 1
     assert torch.isfinite(runner.err).all()
     e = sim.steps_done
     lib = abi.load()
@@ -282,10 +313,10 @@ def _state(sim):
 
 
 def test_graph_blocks_equal_eager_blocks():
-    """Captured propose/commit graphs, replayed over four blocks (all three
-    residues of the step mod 6, one of them twice) with eager work between
-    the replays, leave the state bit for bit equal to eager launches, for
-    accepted and for rejected (fallback) blocks."""
+    """Captured block graphs, replayed over four blocks (all three residues
+    of the step mod 6, one of them twice) with eager work between the
+    replays, leave the state bit for bit equal to eager launches, for
+    accepted blocks and for rejected blocks (gated fallback steps)."""
     wl = params.workload_c1()
     net = _random_net(3)
     for tol in (1e30, 0.0):
@@ -296,9 +327,11 @@ def test_graph_blocks_equal_eager_blocks():
         rb = su.SurrogateRunner(sb, net, tol=tol, graphs=True)
         for b in range(4):
             say(f"tol {tol:g}: block {b} from step {sa.steps_done}")
-            oka, okb = ra.block(), rb.block()
-            assert oka == okb == (tol > 1.0)
-            torch.cuda.synchronize()
+            ra.block()
+            rb.block()
+            sta, stb = ra.stats(), rb.stats()
+            assert sta == stb
+            assert sta["fallback_blocks"] == (0 if tol > 1.0 else b + 1)
             assert sa.steps_done == sb.steps_done
             for x, y in zip(_state(sa), _state(sb)):
                 assert torch.equal(x, y)
@@ -306,4 +339,23 @@ def test_graph_blocks_equal_eager_blocks():
             # eager work between replays
             sb.S_slot(0).sum()
         assert not sa.fatal()
+
+
+def test_rejected_block_equals_plain_steps():
+    """A rejected block (tolerance 0) leaves exactly the state of K plain
+    steps: the gated commit launches change nothing and the gated fallback
+    steps are the plain step."""
+    wl = params.workload_c1()
+    net = _random_net(3)
+    sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
+    sa.step(12)
+    sb.step(12)
+    r = su.SurrogateRunner(sa, net, tol=0.0)
+    r.block()
+    sb.step(8)
+    torch.cuda.synchronize()
+    assert r.stats()["fallback_blocks"] == 1
+    assert sa.steps_done == sb.steps_done
+    for x, y in zip(_state(sa), _state(sb)):
+        assert torch.equal(x, y)
 
