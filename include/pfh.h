@@ -440,10 +440,20 @@ int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic,
 /* Whole-network inference in one kernel (activations stay in LDS): X as for
  * pfh_sur_conv (8 channels, the first 4 used), W1 [32][9][8], W2 and W3
  * [32][9][32] bf16, biases fp32, W4 [9][32] fp32 (applied as bf16 hi + lo).
- * pred = phic + dphi; dphi may be NULL. X, W2 and W3 16-byte aligned. */
+ * pred = phic + dphi; dphi may be NULL. X, W2 and W3 16-byte aligned. Does
+ * nothing when go is non-NULL and *go == 0. */
 int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
                   const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
-                  const float* phic, float* dphi, float* pred, int ncrops, void* stream);
+                  const float* phic, float* dphi, float* pred, int ncrops, const int* go,
+                  void* stream);
+/* pfh_sur_infer with the inputs built in the kernel from the field, the
+ * aggregate and the cell states (the crops of pfh_sur_crop, same values):
+ * writes the crop origins and pred = phi crop + dphi; dphi may be NULL. */
+int pfh_sur_infer_cells(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                        const uint32_t* S, int N, int L, int P, const uint16_t* W1,
+                        const float* b1, const uint16_t* W2, const float* b2, const uint16_t* W3,
+                        const float* b3, const float* W4, float b4, int* origin, float* dphi,
+                        float* pred, const int* go, void* stream);
 /* y = bfloat16(x) with the conversion pfh_sur_infer applies to hidden
  * activations (n even, y 4-byte aligned). */
 int pfh_sur_round_bf16(const float* x, uint16_t* y, int n, void* stream);
@@ -474,10 +484,11 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
                       void* stream);
 /* Acceptance test: err[n] = relative L2 of pred against the check step's
  * field on the recorded crop windows; rejected[0] += cells with err > tol;
- * err_max (may be NULL) = running maximum of err as float bits. */
+ * err_max (may be NULL) = running maximum of err as float bits. Does nothing
+ * when go is non-NULL and *go == 0. */
 int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
                   const int* origin, int N, int L, const float* pred, float tol, float* err,
-                  uint32_t* rejected, uint32_t* err_max, void* stream);
+                  uint32_t* rejected, uint32_t* err_max, const int* go, void* stream);
 /* Accepted block: prediction into every cell's window, q(phi) scattered into
  * the zeroed aggregate S, moments as pfh2d_init_moments computes them.
  * Does nothing when skip_if is non-NULL and *skip_if != 0. */
@@ -488,9 +499,19 @@ int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const 
 int pfh_sur_zero2(uint32_t* a, uint32_t* b, unsigned long long words, const int* skip_if,
                   void* stream);
 /* End of a block: when *rejected == 0 the step counters advance by K and the
- * queue cursors restart; otherwise *rejected_blocks (may be NULL) += 1. */
-int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, const int* rejected,
-                    uint32_t* rejected_blocks, void* stream);
+ * queue cursors restart; otherwise, if *go != 0, *go = 0 (every later gated
+ * launch does nothing), *abort_step = the block's first step (stepc[pin])
+ * and *rejected_blocks (may be NULL) += 1. */
+int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin, int* go,
+                    const int* rejected, long long* abort_step, uint32_t* rejected_blocks,
+                    void* stream);
+/* Start of a block (nothing when *go == 0): copies of the cell states and
+ * classes for the check step, its aggregate zeroed, cursors zeroed, both
+ * step slots = *step_now, *rejected = 0. */
+int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,
+                 pfh_cell_state* cells_chk, const uint8_t* cls, uint8_t* cls_chk, int N,
+                 uint32_t* S_chk, unsigned long long words, unsigned long long* cur_chk,
+                 long long* step_chk, const long long* step_now, void* stream);
 /* The polarity draws of K skipped plain steps, steps *step0 .. *step0+K-1
  * (src/kernels.cu:98-114): theta and tumble_ctr of every cell. step0 is a
  * device pointer (the accepted-step counter), so the launch can be captured

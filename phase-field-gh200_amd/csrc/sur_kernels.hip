@@ -344,13 +344,60 @@ constexpr int kRowX = kHaloCols * kXP;
 constexpr int kRowP = 9 * kHaloCols;           // P ring row: [tap][column] fp32
 constexpr int kFirstStep = -3, kLastStep = 52;
 
+// Where a crop's input comes from when the kernel builds it itself
+struct CropCtx {
+    int ox, oy;           // crop origin (world pixel of crop (0, 0))
+    int gx0, gy0;         // window origin
+    int wx, wy, tx0, ty0; // window shape and its place in the tile
+    uint32_t cs;          // bf16 cos theta | bf16 sin theta << 16
+};
+
 struct InferLDS {
     bf16_t h1[kRing * kRowH];
     bf16_t h2[kRing * kRowH];
     bf16_t x[kRing * kRowX];
     float p[kRing * kRowP];
     float bias[3 * kCH];
+    CropCtx ctx;
 };
+
+struct InferArgs {
+    // inputs as buffers (pfh_sur_conv layout) ...
+    const bf16_t* X;
+    const float* phic;
+    // ... or built from the field (k_sur_crop's definition)
+    const float* phi;
+    const pfh_cell_state* cells;
+    const uint8_t* cls;
+    const uint32_t* S;
+    int L, P;
+    int* origin;
+    // network
+    const bf16_t *W1, *W2, *W3;
+    const float *b1, *b2, *b3, *W4;
+    float b4;
+    float* dphi;
+    float* pred;
+    int ncrops;
+    const int* go;        // NULL, or nothing is done unless *go != 0
+};
+
+// phi of the crop at crop pixel (x, y): the cell's active window, else 0
+__device__ __forceinline__ float ctx_phi(const float* tile, const CropCtx& c, int L, int x, int y,
+                                         int* wxo, int* wyo) {
+    int wx = c.ox + x;
+    if (wx >= L) wx -= L;
+    int wy = c.oy + y;
+    if (wy >= L) wy -= L;
+    *wxo = wx;
+    *wyo = wy;
+    int a = wx - c.gx0;
+    if (a < 0) a += L;
+    int b = wy - c.gy0;
+    if (b < 0) b += L;
+    return (a < c.wx && b < c.wy) ? tile[(size_t)(c.ty0 + b) * pfh::kTilePitch + c.tx0 + a]
+                                  : 0.0f;
+}
 
 __device__ __forceinline__ int ring(int row) { return (row + 16 * kRing) % kRing; }
 __device__ __forceinline__ bool in_crop(int row) { return (unsigned)row < (unsigned)kCrop; }
@@ -421,11 +468,16 @@ __device__ __forceinline__ void store_h(bf16_t* dst, int y, int tile, int col, i
     *reinterpret_cast<u32x4*>(o + 8) = hi;
 }
 
-__global__ __launch_bounds__(kIT, 1) void k_sur_infer(
-    const bf16_t* X, const bf16_t* W1, const float* b1, const bf16_t* W2, const float* b2,
-    const bf16_t* W3, const float* b3, const float* W4, float b4, const float* phic, float* dphi,
-    float* pred, int ncrops) {
+template <bool kCells>
+__global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
     __shared__ InferLDS sm;
+    if (A.go && *A.go == 0) return;
+    const bf16_t *W1 = A.W1, *W2 = A.W2, *W3 = A.W3;
+    const float *b1 = A.b1, *b2 = A.b2, *b3 = A.b3, *W4 = A.W4;
+    const float b4 = A.b4;
+    float* dphi = A.dphi;
+    float* pred = A.pred;
+    const int ncrops = A.ncrops, L = A.L, P = A.P;
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63;
     // wave-uniform: keeps the step's row and ring arithmetic on the scalar unit
@@ -513,24 +565,90 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(
     const int iorow = io / kCrop, iox = io - iorow * kCrop;
 
     for (int crop = (int)blockIdx.x; crop < ncrops; crop += (int)gridDim.x) {
-        const bf16_t* Xc = X + (size_t)crop * kCropPix * kC0;
-        const float* phicc = phic + (size_t)crop * kCropPix;
-        u32x2 xreg = {0u, 0u};      // X rows of the first step are outside the crop
+        const bf16_t* Xc = kCells ? nullptr : A.X + (size_t)crop * kCropPix * kC0;
+        const float* phicc = kCells ? nullptr : A.phic + (size_t)crop * kCropPix;
+        const float* tile = kCells ? A.phi + (size_t)crop * pfh::kTileArea : nullptr;
+        if (kCells) {
+            // the crop of k_sur_crop: centred on the rounded centroid
+            if (tid == 0) {
+                const pfh_cell_state c = A.cells[crop];
+                const pfh::Shape sh = pfh::shape_of((int)A.cls[crop]);
+                CropCtx x;
+                crop_origin(c, L, &x.ox, &x.oy);
+                x.gx0 = c.gx0;
+                x.gy0 = c.gy0;
+                x.wx = sh.wx;
+                x.wy = sh.wy;
+                x.tx0 = sh.tx0;
+                x.ty0 = sh.ty0;
+                double sn, cs;
+                pfh::sincos_d((double)c.theta, &sn, &cs);
+                x.cs = (uint32_t)f2bf((float)cs) | ((uint32_t)f2bf((float)sn) << 16);
+                sm.ctx = x;
+                A.origin[2 * crop] = x.ox;
+                A.origin[2 * crop + 1] = x.oy;
+            }
+            __syncthreads();
+        }
+        // the crop context in scalar registers
+        CropCtx cx = {};
+        if (kCells) {
+            cx.ox = __builtin_amdgcn_readfirstlane(sm.ctx.ox);
+            cx.oy = __builtin_amdgcn_readfirstlane(sm.ctx.oy);
+            cx.gx0 = __builtin_amdgcn_readfirstlane(sm.ctx.gx0);
+            cx.gy0 = __builtin_amdgcn_readfirstlane(sm.ctx.gy0);
+            cx.wx = __builtin_amdgcn_readfirstlane(sm.ctx.wx);
+            cx.wy = __builtin_amdgcn_readfirstlane(sm.ctx.wy);
+            cx.tx0 = __builtin_amdgcn_readfirstlane(sm.ctx.tx0);
+            cx.ty0 = __builtin_amdgcn_readfirstlane(sm.ctx.ty0);
+            cx.cs = (uint32_t)__builtin_amdgcn_readfirstlane((int)sm.ctx.cs);
+        }
+        // X rows of the first step are outside the crop
+        u32x2 xreg = {0u, 0u};
+        float xv = 0.0f;
+        uint32_t xs = 0u;
+        bool xin = false;           // the fetched row lies inside the crop
         float preg = 0.0f;
         for (int s = kFirstStep; s <= kLastStep; ++s) {
             if (early && wave >= 3) {
                 // X rows [2s+3, 2s+5): store what step s-1 fetched, fetch [2s+5, 2s+7)
                 const int yw = 2 * s + 3 + iorow;
+                if (kCells) {
+                    // phi, S_other = (S - q(phi)) 2^-27 exact (0 when S < q),
+                    // cos theta, sin theta
+                    // rows outside the crop are zero in every channel
+                    const uint32_t q = pfh::q_of(xv);
+                    const float other = xs >= q ? (float)(xs - q) * pfh::kQInv : 0.0f;
+                    xreg = xin ? u32x2{pack_bf16(xv, other), cx.cs} : u32x2{0u, 0u};
+                }
                 *reinterpret_cast<u32x2*>(sm.x + ring(yw) * kRowX + (iox + 1) * kXP) = xreg;
                 const int yf = yw + 2;
-                xreg = in_crop(yf)
-                           ? *reinterpret_cast<const u32x2*>(Xc + ((size_t)yf * kCrop + iox) * kC0)
-                           : u32x2{0u, 0u};
+                if (kCells) {
+                    xv = 0.0f;
+                    xs = 0u;
+                    xin = in_crop(yf);
+                    if (xin) {
+                        int wxp, wyp;
+                        xv = ctx_phi(tile, cx, L, iox, yf, &wxp, &wyp);
+                        xs = A.S[(size_t)wyp * P + wxp];
+                    }
+                } else {
+                    xreg = in_crop(yf) ? *reinterpret_cast<const u32x2*>(
+                                             Xc + ((size_t)yf * kCrop + iox) * kC0)
+                                       : u32x2{0u, 0u};
+                }
             }
             float pnext = 0.0f;
             if (early && wave < 3) {
                 const int yn = 2 * s - 7 + iorow;     // output row of step s+1
-                if (in_crop(yn)) pnext = phicc[yn * kCrop + iox];
+                if (in_crop(yn)) {
+                    if (kCells) {
+                        int wxp, wyp;
+                        pnext = ctx_phi(tile, cx, L, iox, yn, &wxp, &wyp);
+                    } else {
+                        pnext = phicc[yn * kCrop + iox];
+                    }
+                }
             }
             if (early) {
                 // conv1: h1 row 2s + jrow
@@ -802,7 +920,9 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
                                                    const pfh_cell_state* cells,
                                                    const uint8_t* cls, const int* origin, int L,
                                                    const float* pred, float tol, float* err,
-                                                   uint32_t* rejected, uint32_t* err_max) {
+                                                   uint32_t* rejected, uint32_t* err_max,
+                                                   const int* go) {
+    if (go && *go == 0) return;
     __shared__ double r1[4], r2[4];
     const int n = (int)blockIdx.x;
     const pfh_cell_state c = cells[n];
@@ -921,20 +1041,57 @@ __global__ void k_sur_zero2(uint32_t* a, uint32_t* b, size_t words, const int* s
     }
 }
 
-// End of a block: accepted -> the step counters advance by K and the queue
-// cursors restart; rejected -> the gated plain steps that follow advance
-// them, and rejected_blocks counts the block.
-__global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int K,
-                              const int* rejected, uint32_t* rejected_blocks) {
+// End of a block. Accepted: the step counters advance by K and the queue
+// cursors restart. Rejected while running: *go drops to 0 (every later
+// gated launch does nothing), abort_step records the block's first step and
+// rejected_blocks counts it; the caller then runs the plain steps itself.
+__global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin,
+                              int* go, const int* rejected, long long* abort_step,
+                              uint32_t* rejected_blocks) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (*rejected) {
-        if (rejected_blocks) *rejected_blocks += 1u;
+        if (*go) {
+            *go = 0;
+            *abort_step = stepc[pin];
+            if (rejected_blocks) *rejected_blocks += 1u;
+        }
         return;
     }
     stepc[0] += K;
     stepc[1] += K;
     cursor[0] = 0ull;
     cursor[1] = 0ull;
+}
+
+// Start of a block (nothing when *go == 0): the check step's private copies
+// of the cell states and classes, its zeroed aggregate, queue cursors and
+// step counter, and the rejected-cell count reset.
+__global__ __launch_bounds__(256) void k_sur_prep(const int* go, int* rejected,
+                                                  const pfh_cell_state* cells,
+                                                  pfh_cell_state* cells_chk, const uint8_t* cls,
+                                                  uint8_t* cls_chk, int N, uint32_t* S_chk,
+                                                  size_t words, unsigned long long* cur_chk,
+                                                  long long* step_chk, const long long* step_now) {
+    if (*go == 0) return;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    u32x4* sv = reinterpret_cast<u32x4*>(S_chk);
+    for (size_t i = i0; i < (words >> 2); i += stride) sv[i] = z;
+    for (size_t i = ((words >> 2) << 2) + i0; i < words; i += stride) S_chk[i] = 0u;
+    constexpr size_t kCellVec = sizeof(pfh_cell_state) / sizeof(u32x4);
+    const u32x4* cs = reinterpret_cast<const u32x4*>(cells);
+    u32x4* cd = reinterpret_cast<u32x4*>(cells_chk);
+    for (size_t i = i0; i < (size_t)N * kCellVec; i += stride) cd[i] = cs[i];
+    for (size_t i = i0; i < (size_t)N; i += stride) cls_chk[i] = cls[i];
+    if (i0 == 0) {
+        const long long st = *step_now;
+        cur_chk[0] = 0ull;
+        cur_chk[1] = 0ull;
+        step_chk[0] = st;
+        step_chk[1] = st;
+        *rejected = 0;
+    }
 }
 
 }  // namespace sur
@@ -997,22 +1154,72 @@ int pfh_sur_out(const uint16_t* H, const float* W4, float b4, const float* phic,
     return launch_status("k_sur_out");
 }
 
-int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
-                  const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
-                  const float* phic, float* dphi, float* pred, int ncrops, void* stream) {
-    if (!X || !W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !W4 || !phic || !pred || ncrops <= 0)
-        return invalid("sur_infer");
-    if (((uintptr_t)X | (uintptr_t)W2 | (uintptr_t)W3) & 15u) return invalid("sur_infer alignment");
+namespace {
+int infer_launch(const sur::InferArgs& a, bool cells, void* stream) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    const int grid = ncrops < cus ? ncrops : cus;
-    hipLaunchKernelGGL(sur::k_sur_infer, dim3((unsigned)grid), dim3(sur::kIT), 0,
-                       as_stream(stream), X, W1, b1, W2, b2, W3, b3, W4, b4, phic, dphi, pred,
-                       ncrops);
+    const int grid = a.ncrops < cus ? a.ncrops : cus;
+    if (cells)
+        hipLaunchKernelGGL(sur::k_sur_infer<true>, dim3((unsigned)grid), dim3(sur::kIT), 0,
+                           as_stream(stream), a);
+    else
+        hipLaunchKernelGGL(sur::k_sur_infer<false>, dim3((unsigned)grid), dim3(sur::kIT), 0,
+                           as_stream(stream), a);
     return launch_status("k_sur_infer");
+}
+
+bool weights_ok(const uint16_t* W1, const float* b1, const uint16_t* W2, const float* b2,
+                const uint16_t* W3, const float* b3, const float* W4) {
+    return W1 && b1 && W2 && b2 && W3 && b3 && W4 &&
+           !((((uintptr_t)W2) | ((uintptr_t)W3)) & 15u);
+}
+}  // namespace
+
+int pfh_sur_infer(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
+                  const float* b2, const uint16_t* W3, const float* b3, const float* W4, float b4,
+                  const float* phic, float* dphi, float* pred, int ncrops, const int* go,
+                  void* stream) {
+    if (!X || !phic || !pred || ncrops <= 0 || !weights_ok(W1, b1, W2, b2, W3, b3, W4) ||
+        (((uintptr_t)X) & 15u))
+        return invalid("sur_infer");
+    sur::InferArgs a = {};
+    a.X = X;
+    a.phic = phic;
+    a.W1 = W1; a.W2 = W2; a.W3 = W3;
+    a.b1 = b1; a.b2 = b2; a.b3 = b3; a.W4 = W4; a.b4 = b4;
+    a.dphi = dphi;
+    a.pred = pred;
+    a.ncrops = ncrops;
+    a.go = go;
+    return infer_launch(a, false, stream);
+}
+
+int pfh_sur_infer_cells(const float* phi, const pfh_cell_state* cells, const uint8_t* cls,
+                        const uint32_t* S, int N, int L, int P, const uint16_t* W1,
+                        const float* b1, const uint16_t* W2, const float* b2, const uint16_t* W3,
+                        const float* b3, const float* W4, float b4, int* origin, float* dphi,
+                        float* pred, const int* go, void* stream) {
+    if (!phi || !cells || !cls || !S || !origin || !pred || N <= 0 || L < sur::kCrop ||
+        P < L || !weights_ok(W1, b1, W2, b2, W3, b3, W4))
+        return invalid("sur_infer_cells");
+    sur::InferArgs a = {};
+    a.phi = phi;
+    a.cells = cells;
+    a.cls = cls;
+    a.S = S;
+    a.L = L;
+    a.P = P;
+    a.origin = origin;
+    a.W1 = W1; a.W2 = W2; a.W3 = W3;
+    a.b1 = b1; a.b2 = b2; a.b3 = b3; a.W4 = W4; a.b4 = b4;
+    a.dphi = dphi;
+    a.pred = pred;
+    a.ncrops = N;
+    a.go = go;
+    return infer_launch(a, true, stream);
 }
 
 int pfh_sur_round_bf16(const float* x, uint16_t* y, int n, void* stream) {
@@ -1092,12 +1299,12 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
 
 int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
                   const int* origin, int N, int L, const float* pred, float tol, float* err,
-                  uint32_t* rejected, uint32_t* err_max, void* stream) {
+                  uint32_t* rejected, uint32_t* err_max, const int* go, void* stream) {
     if (!phi_chk || !cells || !cls || !origin || !pred || !err || !rejected || N <= 0 ||
         L < sur::kCrop)
         return invalid("sur_check");
     hipLaunchKernelGGL(sur::k_sur_check, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
-                       phi_chk, cells, cls, origin, L, pred, tol, err, rejected, err_max);
+                       phi_chk, cells, cls, origin, L, pred, tol, err, rejected, err_max, go);
     return launch_status("k_sur_check");
 }
 
@@ -1137,12 +1344,33 @@ int pfh_sur_zero2(uint32_t* a, uint32_t* b, unsigned long long words, const int*
     return launch_status("k_sur_zero2");
 }
 
-int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, const int* rejected,
-                    uint32_t* rejected_blocks, void* stream) {
-    if (!stepc || !cursor || !rejected || K <= 0) return invalid("sur_advance");
+int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin, int* go,
+                    const int* rejected, long long* abort_step, uint32_t* rejected_blocks,
+                    void* stream) {
+    if (!stepc || !cursor || !go || !rejected || !abort_step || K <= 0 || (pin & ~1))
+        return invalid("sur_advance");
     hipLaunchKernelGGL(sur::k_sur_advance, dim3(1), dim3(64), 0, as_stream(stream), stepc, cursor,
-                       K, rejected, rejected_blocks);
+                       K, pin, go, rejected, abort_step, rejected_blocks);
     return launch_status("k_sur_advance");
+}
+
+int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,
+                 pfh_cell_state* cells_chk, const uint8_t* cls, uint8_t* cls_chk, int N,
+                 uint32_t* S_chk, unsigned long long words, unsigned long long* cur_chk,
+                 long long* step_chk, const long long* step_now, void* stream) {
+    if (!go || !rejected || !cells || !cells_chk || !cls || !cls_chk || N <= 0 || !S_chk ||
+        !cur_chk || !step_chk || !step_now ||
+        ((((uintptr_t)cells) | ((uintptr_t)cells_chk) | ((uintptr_t)S_chk)) & 15u))
+        return invalid("sur_prep");
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    hipLaunchKernelGGL(sur::k_sur_prep, dim3((unsigned)(4 * cus)), dim3(256), 0,
+                       as_stream(stream), go, rejected, cells, cells_chk, cls, cls_chk, N, S_chk,
+                       (size_t)words, cur_chk, step_chk, step_now);
+    return launch_status("k_sur_prep");
 }
 
 }  // extern "C"

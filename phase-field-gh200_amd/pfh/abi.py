@@ -181,12 +181,15 @@ SIGNATURES = {
     "pfh_sur_recrop": (_I, [_P, _P, _P, _P, _I, _I, _P, _P]),
     "pfh_sur_conv": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _P]),
     "pfh_sur_out": (_I, [_P, _P, _F, _P, _P, _P, _I, _P]),
-    "pfh_sur_check": (_I, [_P, _P, _P, _P, _I, _I, _P, _F, _P, _P, _P, _P]),
+    "pfh_sur_check": (_I, [_P, _P, _P, _P, _I, _I, _P, _F, _P, _P, _P, _P, _P]),
     "pfh_sur_commit": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "pfh_sur_zero2": (_I, [_P, _P, C.c_ulonglong, _P, _P]),
-    "pfh_sur_advance": (_I, [_P, _P, _I, _P, _P, _P]),
+    "pfh_sur_advance": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "pfh_sur_prep": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, C.c_ulonglong, _P, _P, _P, _P]),
     "pfh_sur_round_bf16": (_I, [_P, _P, _I, _P]),
-    "pfh_sur_infer": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _I, _P]),
+    "pfh_sur_infer_cells": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _F,
+                                _P, _P, _P, _P, _P]),
+    "pfh_sur_infer": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _I, _P, _P]),
     "pfh_sur_loss": (_I, [_P, _P, _P, _P, _I, _F, _P]),
     "pfh_sur_out_bwd": (_I, [_P, _P, _P, _P, _P, _I, _P]),
     "pfh_sur_wgrad": (_I, [_I, _P, _P, _P, _I, _P]),

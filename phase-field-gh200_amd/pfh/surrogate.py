@@ -164,7 +164,7 @@ def forward(net: Net, ws: Workspace, n: int, X0=None, phic=None, pred=True, stre
               "out")
 
 
-def infer(net: Net, ws: Workspace, n: int, W4=None, b4=None, dphi=True, stream=None):
+def infer(net: Net, ws: Workspace, n: int, W4=None, b4=None, dphi=True, stream=None, go=None):
     """The whole network in one kernel (pfh_sur_infer): ws.pred = ws.phic +
     dphi for n crops, and ws.dphi when `dphi`. W4/b4 default to the raw
     output layer (no scale)."""
@@ -174,8 +174,26 @@ def infer(net: Net, ws: Workspace, n: int, W4=None, b4=None, dphi=True, stream=N
     abi.check(net.lib.pfh_sur_infer(_ptr(ws.X0), _ptr(net.wb["W1"]), _ptr(b["b1"]),
                                     _ptr(net.wb["W2"]), _ptr(b["b2"]), _ptr(net.wb["W3"]),
                                     _ptr(b["b3"]), _ptr(W4), float(b4), _ptr(ws.phic),
-                                    _ptr(ws.dphi) if dphi else None, _ptr(ws.pred), n, stream),
+                                    _ptr(ws.dphi) if dphi else None, _ptr(ws.pred), n,
+                                    None if go is None else _ptr(go), stream),
               "pfh_sur_infer")
+
+
+def infer_cells(net: Net, sim, ws: Workspace, W4=None, b4=None, dphi=True, stream=None,
+                go=None):
+    """infer() with the crops built inside the kernel from the Sim2D's field,
+    aggregate and cell states (pfh_sur_infer_cells): ws.origin, ws.pred and,
+    when `dphi`, ws.dphi; ws.X0 and ws.phic are not used."""
+    W4 = net.views["W4"] if W4 is None else W4
+    b4 = net.b4() if b4 is None else b4
+    b = net.views
+    s = sim.steps_done
+    abi.check(net.lib.pfh_sur_infer_cells(
+        _ptr(sim.phi[s % 2]), _ptr(sim.cell), _ptr(sim.cls), sim._s_ptr(s % 3), sim.N, sim.L,
+        sim.P, _ptr(net.wb["W1"]), _ptr(b["b1"]), _ptr(net.wb["W2"]), _ptr(b["b2"]),
+        _ptr(net.wb["W3"]), _ptr(b["b3"]), _ptr(W4), float(b4), _ptr(ws.origin),
+        _ptr(ws.dphi) if dphi else None, _ptr(ws.pred), None if go is None else _ptr(go),
+        sim._sp() if stream is None else stream), "pfh_sur_infer_cells")
 
 
 def backward(net: Net, ws: Workspace, n: int, target: torch.Tensor, X0=None, stream=None):
@@ -322,23 +340,27 @@ def deploy_weights(net: Net) -> tuple[torch.Tensor, float]:
 # ---------------------------------------------------------------------------
 class SurrogateRunner:
     """Advances a Sim2D K steps at a time with the surrogate, checked per
-    cell against one explicit step of size K dt, falling back to K plain
-    steps when any cell is rejected (whole-population fallback).
+    cell against one explicit step of size K dt; a block with any rejected
+    cell is replaced by K plain steps (whole-population fallback).
 
-    A block is issued as one stream of launches with no host round trip:
-    propose (crops, inference, check step, acceptance test into a device
-    rejected-cell count), then the commit launches (polarity draws, exact S
-    and moments, counters advanced by K), which do nothing when the count is
-    non-zero, then K plain steps gated on the same count (`run_if`), which do
-    nothing when it is zero. Either way the state advances K steps, so the
-    next block is issued at once. With `graphs` the launches of a block are
+    Blocks are issued in chunks with no host round trip inside a chunk. Every
+    launch of a block is gated on a device flag `go`: propose (check-step
+    copies, fused inference with in-kernel crops, the check step, the
+    acceptance test into a rejected-cell count), then commit (polarity
+    draws, exact S and moments, counters advanced by K), which does nothing
+    when the count is non-zero. A rejection clears `go` and records the
+    block's first step, so the rest of the chunk does nothing; after the
+    chunk the host runs the K plain steps of that block, sets `go` again and
+    issues the remaining blocks. With `graphs` the launches of a block are
     captured once per residue of the step index mod 6 (the phi parity and S
-    rotation it touches) and replayed. stats() reads the outcome counters."""
+    rotation it touches) and replayed."""
 
-    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3, graphs: bool = True):
+    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3, graphs: bool = True,
+                 chunk: int = 8):
         if K <= 0 or K % 2:
             raise ValueError("K must be a positive even number of steps")
         self.sim, self.net, self.K, self.tol = sim, net, K, tol
+        self.chunk = max(1, int(chunk))
         self.lib = abi.load()
         dev = sim.dev
         N = sim.N
@@ -352,12 +374,13 @@ class SurrogateRunner:
         self.step_chk = torch.zeros(2, dtype=torch.int64, device=dev)
         self.flags_chk = torch.zeros_like(sim.flags)
         self.err = torch.zeros(N, dtype=torch.float32, device=dev)
-        self.rejected = torch.zeros(1, dtype=torch.int32, device=dev)
+        # [0] go, [1] rejected cells of the current block
+        self.ctl = torch.tensor([1, 0], dtype=torch.int32, device=dev)
+        self.abort_step = torch.zeros(1, dtype=torch.int64, device=dev)
         # [0] largest check error so far (float bits), [1] rejected blocks
         self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
         self.graphs = graphs
         self._g = {}
-        self._fb = [self._fallback_args(k) for k in range(6)]
         self.blocks = 0
 
     def _check_args(self, slot: int) -> abi.StepArgs:
@@ -377,14 +400,7 @@ class SurrogateRunner:
         a.flags = _ptr(self.flags_chk)
         a.dt = float(np.float32(sim.p.dt * self.K))
         a.physical_dt = sim.p.dt * self.K
-        return a
-
-    def _fallback_args(self, slot: int) -> abi.StepArgs:
-        """The plain step of `slot`, gated on the rejected-cell count."""
-        sim = self.sim
-        a = abi.StepArgs()
-        C.memmove(C.addressof(a), C.addressof(sim._args[slot]), C.sizeof(a))
-        a.run_if = _ptr(self.rejected)
+        a.run_if = _ptr(self.ctl)
         return a
 
     def _issue(self, s: int) -> None:
@@ -394,39 +410,34 @@ class SurrogateRunner:
         N = sim.N
         pin = s % 2
         e = s + K
-        rej = _ptr(self.rejected)
-        # propose: crops, inference, one plain step of K dt from phi_n on
-        # copies of the state, acceptance test
-        crop_sim(sim, self.ws)
-        infer(self.net, self.ws, N, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp)
-        self.cell_chk.copy_(sim.cell)
-        self.cls_chk.copy_(sim.cls)
-        self.S_chk[:sim.words].zero_()
-        self.cur_chk.zero_()
-        self.step_chk.copy_(sim.stepc[pin:pin + 1].expand(2))
+        go = _ptr(self.ctl)
+        rej = go + 4
+        now = _ptr(sim.stepc) + 8 * pin
+        # propose
+        abi.check(lib.pfh_sur_prep(go, rej, _ptr(sim.cell), _ptr(self.cell_chk), _ptr(sim.cls),
+                                   _ptr(self.cls_chk), N, _ptr(self.S_chk), sim.words,
+                                   _ptr(self.cur_chk), _ptr(self.step_chk), now, sp), "prep")
+        infer_cells(self.net, sim, self.ws, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp,
+                    go=self.ctl)
         a = self._check_args(s)
         abi.check(lib.pfh2d_launch_step(C.byref(a), sim.grid, sp, None, 0, 0.0, 0), "check step")
-        self.rejected.zero_()
         abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
-                                    self.tol, _ptr(self.err), rej, _ptr(self.counters), sp),
+                                    self.tol, _ptr(self.err), rej, _ptr(self.counters), go, sp),
                   "check")
-        # commit (no effect when rejected): the next step reads S[e%3] and
-        # scatters into S[(e+1)%3]; it clears S[(e+2)%3] ahead by itself
+        # commit (nothing when the block has rejected cells): the next step
+        # reads S[e%3] and scatters into S[(e+1)%3]; it clears S[(e+2)%3]
+        # ahead by itself
         abi.check(lib.pfh_sur_zero2(sim._s_ptr(e % 3), sim._s_ptr((e + 1) % 3), sim.words, rej,
                                     sp), "zero2")
-        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, _ptr(sim.stepc) + 8 * pin, K,
-                                     sim.p.polarity_stream(), sim.p.p_tumble(), rej, sp),
-                  "tumble")
+        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, now, K, sim.p.polarity_stream(),
+                                     sim.p.p_tumble(), rej, sp), "tumble")
         abi.check(lib.pfh_sur_commit(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
                                      _ptr(self.ws.origin), _ptr(self.ws.pred), N, sim.L, sim.P,
                                      sim._s_ptr(e % 3), _ptr(sim.flags), rej, sp), "commit")
-        abi.check(lib.pfh_sur_advance(_ptr(sim.stepc), _ptr(sim.cursor), K, rej,
-                                      _ptr(self.counters) + 4, sp), "advance")
-        # fallback (no effect when accepted): K plain steps
-        for k in range(K):
-            abi.check(lib.pfh2d_launch_step(C.byref(self._fb[(s + k) % 6]), sim.grid, sp, None,
-                                            0, 0.0, 0), "fallback step")
+        abi.check(lib.pfh_sur_advance(_ptr(sim.stepc), _ptr(sim.cursor), K, pin, go, rej,
+                                      _ptr(self.abort_step), _ptr(self.counters) + 4, sp),
+                  "advance")
 
     def _captured(self, r: int) -> torch.cuda.CUDAGraph:
         g = self._g.get(r)
@@ -456,8 +467,7 @@ class SurrogateRunner:
             for b in range(3):
                 self._captured((self.sim.steps_done + b * self.K) % 6)
 
-    def block(self) -> None:
-        """Issue the next K steps (surrogate, or plain when rejected)."""
+    def _launch(self) -> None:
         sim = self.sim
         s = sim.steps_done
         if self.graphs:
@@ -465,17 +475,40 @@ class SurrogateRunner:
         else:
             self._issue(s)
         sim.steps_done = s + self.K
-        self.blocks += 1
+
+    def run(self, steps: int) -> None:
+        """Advance `steps` (a multiple of K) steps."""
+        sim, K = self.sim, self.K
+        if steps % K:
+            raise ValueError(f"steps must be a multiple of K = {K}")
+        left = steps // K
+        while left:
+            m = min(self.chunk, left)
+            s0 = sim.steps_done
+            for _ in range(m):
+                self._launch()
+            self.blocks += m
+            if int(self.ctl[0].item()):           # waits for the chunk
+                left -= m
+                continue
+            # the block at abort_step was rejected and the blocks after it
+            # did nothing: run its plain steps, then carry on from there
+            sa = int(self.abort_step.item())
+            done = (sa - s0) // K
+            self.blocks -= m - done - 1
+            sim.steps_done = sa
+            sim.run(K)
+            self.ctl.fill_(1)
+            left -= done + 1
+
+    def block(self) -> None:
+        """Advance K steps (surrogate, or plain when rejected)."""
+        self.run(self.K)
 
     def stats(self) -> dict:
-        """Blocks issued, rejected (fallback) blocks and the largest
-        per-cell check error so far (waits for the stream)."""
+        """Blocks run, rejected (fallback) blocks and the largest per-cell
+        check error so far (waits for the stream)."""
         c = self.counters.cpu().numpy()
         return {"blocks": self.blocks, "fallback_blocks": int(c[1]),
                 "accepted_blocks": self.blocks - int(c[1]),
                 "max_check_error": float(c[:1].view(np.float32)[0])}
-
-    def run(self, steps: int) -> None:
-        assert steps % self.K == 0
-        for _ in range(steps // self.K):
-            self.block()

@@ -95,8 +95,7 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
     t0 = time.perf_counter()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s0.record(sur.stream)
-    for _ in range(blocks):
-        runner.block()
+    runner.run(blocks * K)
     s1.record(sur.stream)
     torch.cuda.synchronize()
     st = runner.stats()
@@ -129,18 +128,19 @@ INFER_FLOP_PER_CROP = 2 * su.PIX * 9 * (4 * su.CH + su.CH * su.CH * 2 + su.CH)
 
 
 def time_infer(runner: su.SurrogateRunner, reps: int = 20) -> dict:
-    """Average duration of the fused inference kernel on the runner's
-    stream (inputs of the last block) and its MFMA rate."""
+    """Average duration of the fused inference kernel (crops built in the
+    kernel, as in a block) on the runner's stream, on the current state, and
+    its MFMA rate."""
     sim = runner.sim
     sp = sim._sp()
     stream = sim.stream
     ws = runner.ws
-    su.infer(runner.net, ws, sim.N, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
+    su.infer_cells(runner.net, sim, ws, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(reps):
-        su.infer(runner.net, ws, sim.N, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
+        su.infer_cells(runner.net, sim, ws, W4=runner.W4d, b4=runner.b4d, dphi=False, stream=sp)
     e1.record(stream)
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps

@@ -120,6 +120,23 @@ def test_forward_matches_oracle(R, c1_after):
     assert rel <= 1e-2
 
 
+def test_fused_inference_from_cells_equals_buffers(c1_after):
+    """pfh_sur_infer_cells (crops built inside the kernel) gives bit for bit
+    the origins and predictions of pfh_sur_crop followed by pfh_sur_infer."""
+    sim = c1_after
+    net = _random_net(7)
+    a = su.Workspace(sim.N, sim.dev, hidden=False)
+    su.crop_sim(sim, a)
+    su.infer(net, a, sim.N)
+    b = su.Workspace(sim.N, sim.dev, hidden=False)
+    b.pred.fill_(float("nan"))
+    su.infer_cells(net, sim, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a.origin, b.origin)
+    assert torch.equal(a.dphi, b.dphi)
+    assert torch.equal(a.pred, b.pred)
+
+
 def _w4_hi_lo(w4):
     """The fused kernel's output-layer weights: bf16 hi + bf16 lo parts."""
     from oracle import sur_ref as R
@@ -343,19 +360,58 @@ def test_graph_blocks_equal_eager_blocks():
 
 def test_rejected_block_equals_plain_steps():
     """A rejected block (tolerance 0) leaves exactly the state of K plain
-    steps: the gated commit launches change nothing and the gated fallback
-    steps are the plain step."""
+    steps: the gated launches change nothing and the host runs the plain
+    steps; four rejected blocks in one chunk equal 32 plain steps."""
     wl = params.workload_c1()
     net = _random_net(3)
     sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
     sa.step(12)
     sb.step(12)
-    r = su.SurrogateRunner(sa, net, tol=0.0)
-    r.block()
-    sb.step(8)
+    r = su.SurrogateRunner(sa, net, tol=0.0, chunk=8)
+    r.run(32)
+    sb.step(32)
     torch.cuda.synchronize()
-    assert r.stats()["fallback_blocks"] == 1
+    st = r.stats()
+    assert st["fallback_blocks"] == 4 and st["blocks"] == 4
     assert sa.steps_done == sb.steps_done
     for x, y in zip(_state(sa), _state(sb)):
         assert torch.equal(x, y)
 
+
+class _SwitchingRunner(su.SurrogateRunner):
+    """Eager runner whose first `n_ok` blocks are accepted and the rest
+    rejected (tolerance switched per issued block)."""
+
+    def __init__(self, *a, n_ok=2, **k):
+        super().__init__(*a, graphs=False, **k)
+        self.n_issued = 0
+        self.n_ok = n_ok
+
+    def _issue(self, s):
+        self.tol = 1e30 if self.n_issued < self.n_ok else 0.0
+        self.n_issued += 1
+        super()._issue(s)
+
+
+def test_abort_inside_chunk():
+    """A rejection in the third block of a four-block chunk: the blocks
+    after it do nothing, the host runs its plain steps and re-issues the
+    last block, and the state equals two accepted surrogate blocks followed
+    by 16 plain steps."""
+    wl = params.workload_c1()
+    net = _random_net(3)
+    sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
+    sa.step(12)
+    sb.step(12)
+    ra = _SwitchingRunner(sa, net, n_ok=2, chunk=4)
+    ra.run(32)
+    rb = su.SurrogateRunner(sb, net, tol=1e30, graphs=False, chunk=1)
+    rb.run(16)
+    sb.step(16)
+    torch.cuda.synchronize()
+    st = ra.stats()
+    say(f"stats {st}")
+    assert st["blocks"] == 4 and st["fallback_blocks"] == 2
+    assert sa.steps_done == sb.steps_done == 12 + 32
+    for x, y in zip(_state(sa), _state(sb)):
+        assert torch.equal(x, y)
