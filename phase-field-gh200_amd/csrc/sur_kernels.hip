@@ -437,20 +437,27 @@ __device__ __forceinline__ f32x16 conv32_tile(const bf16_t* src, const bf16x8* w
     return acc;
 }
 
-// 4 -> 32 convolution (k = 4 tap + channel, 36 of 48 used) of one tile
+// 4 -> 32 convolution (k = 4 tap + channel, 36 of 48 used) of one tile. In
+// k-step s, lane half 0 holds taps 4s, 4s+1 and half 1 taps 4s+2, 4s+3 (4
+// channels each); the taps are compile-time constants per half, the three
+// ring rows are wave-uniform.
 __device__ __forceinline__ f32x16 conv4_tile(const bf16_t* xr, const bf16x8* wa, int y, int tile,
                                              int col, int half) {
+    const int rb0 = ring(y - 1) * kRowX, rb1 = ring(y) * kRowX, rb2 = ring(y + 1) * kRowX;
+    const bf16_t* base = xr + (32 * tile + col) * kXP;
     f32x16 acc = {};
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         uint32_t v[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int t = 4 * s + 2 * half + j;
-            if (t < 9) {
-                const int ky = t / 3, kx = t - 3 * ky;
-                const u32x2 q = *reinterpret_cast<const u32x2*>(
-                    xr + ring(y + ky - 1) * kRowX + (32 * tile + col + kx) * kXP);
+            const int t0 = 4 * s + j, t1 = 4 * s + 2 + j;
+            const int ky0 = t0 / 3, kx0 = t0 % 3, ky1 = t1 / 3, kx1 = t1 % 3;
+            const int o0 = (ky0 == 0 ? rb0 : ky0 == 1 ? rb1 : rb2) + kx0 * kXP;
+            const int o1 = (ky1 == 0 ? rb0 : ky1 == 1 ? rb1 : rb2) + kx1 * kXP;
+            const bool ok = half ? (t1 < 9) : (t0 < 9);
+            if (ok) {
+                const u32x2 q = *reinterpret_cast<const u32x2*>(base + (half ? o1 : o0));
                 v[2 * j] = q[0];
                 v[2 * j + 1] = q[1];
             }
