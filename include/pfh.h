@@ -127,10 +127,6 @@ typedef struct pfh2d_step_args {
     double p_tumble;
     int full_moment_every;
     unsigned long long clear_ahead_words;
-    /* NULL, or the launch does nothing unless *run_if != 0 (no reference
-     * counterpart: lets a captured surrogate block carry its plain-step
-     * fallback, gated on the device-side rejected-cell count). */
-    const int* run_if;
 } pfh2d_step_args;
 
 /* Kernel resource report: include/kernels.cuh:230-244 (KernelStats). */
@@ -193,6 +189,12 @@ int pfh2d_step_grid(int device);
 int pfh2d_launch_step(const pfh2d_step_args* args, int grid, void* stream,
                       const void* l2_base, size_t l2_bytes,
                       float l2_hit_ratio, int observation);
+/* pfh2d_launch_step gated on a device word: the launch does nothing unless
+ * *run_if != 0 (run_if may be NULL: always runs). No reference counterpart:
+ * lets a captured surrogate block carry steps whose execution depends on a
+ * device-side decision (the check step runs only while the block is live). */
+int pfh2d_launch_step_if(const pfh2d_step_args* args, const int* run_if, int grid,
+                         void* stream);
 
 /* Replaces query_kernel_stats (include/kernels.cuh:243-244) for k_step. */
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out);

@@ -858,10 +858,10 @@ __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
 // -------------------------------------------------------------------------
 // The persistent step kernel (src/kernels.cu:948-1030).
 // -------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A) {
+__global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const int* run_if) {
     __shared__ Smem sm;
     const int tid = (int)threadIdx.x;
-    if (A.run_if != nullptr && *A.run_if == 0) return;   // gated-off launch: no effect
+    if (run_if != nullptr && *run_if == 0) return;   // gated-off launch: no effect
 
     // Clear S[(step+2)%3] ahead: this slot is neither read nor scattered
     // into by this launch. Non-temporal stores keep it out of the caches.
@@ -1181,13 +1181,9 @@ int pfh2d_step_grid(int device) {
     return cus > 0 ? cus : 1;
 }
 
-int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
-                      const void* l2_base, size_t l2_bytes, float l2_hit_ratio,
-                      int observation) {
-    (void)l2_base; (void)l2_bytes; (void)l2_hit_ratio;
+namespace {
+int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* stream) {
     if (!a) return invalid("args");
-    if (observation != 0)
-        return unsupported("velocity observation is not part of this build");
     if (a->moments) return invalid("moments must be NULL");
     if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
         !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||
@@ -1199,8 +1195,24 @@ int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
     // (include/params.cuh:666-680), or S self-subtraction is not exact.
     if (a->L <= 286) return invalid("domain side must exceed 286");
     hipLaunchKernelGGL(pfh::k_step, dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                       as_stream(stream), *a);
+                       as_stream(stream), *a, run_if);
     return launch_status("k_step");
+}
+}  // namespace
+
+int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
+                      const void* l2_base, size_t l2_bytes, float l2_hit_ratio,
+                      int observation) {
+    (void)l2_base; (void)l2_bytes; (void)l2_hit_ratio;
+    if (!a) return invalid("args");
+    if (observation != 0)
+        return unsupported("velocity observation is not part of this build");
+    return step_launch(a, nullptr, grid, stream);
+}
+
+int pfh2d_launch_step_if(const pfh2d_step_args* a, const int* run_if, int grid,
+                         void* stream) {
+    return step_launch(a, run_if, grid, stream);
 }
 
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out) {

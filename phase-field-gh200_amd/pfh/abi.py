@@ -68,7 +68,6 @@ class StepArgs(C.Structure):
         ("seed", C.c_ulonglong), ("polarity_seed", C.c_ulonglong),
         ("p_tumble", C.c_double), ("full_moment_every", C.c_int),
         ("clear_ahead_words", C.c_ulonglong),
-        ("run_if", C.c_void_p),
     ]
 
 
@@ -79,7 +78,7 @@ class KernelStats(C.Structure):
                 ("warps_per_sm", C.c_int), ("occupancy", C.c_double)]
 
 
-assert C.sizeof(StepArgs) == 224, C.sizeof(StepArgs)
+assert C.sizeof(StepArgs) == 216, C.sizeof(StepArgs)
 
 # ---- 3D records (include/pf3d/kernels.cuh) ----
 FLAG3_COUNT = 10
@@ -154,6 +153,7 @@ SIGNATURES = {
     "pfh2d_configure_morton": (_I, [_I]),
     "pfh2d_step_grid": (_I, [_I]),
     "pfh2d_launch_step": (_I, [C.POINTER(StepArgs), _I, _P, _P, _SZ, _F, _I]),
+    "pfh2d_launch_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
     "pfh2d_query_step_stats": (_I, [_I, C.POINTER(KernelStats)]),
     "pfh2d_init_tiles": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _F, _F, _P]),
     "pfh2d_init_moments": (_I, [_P, _P, _P, _I, _P]),

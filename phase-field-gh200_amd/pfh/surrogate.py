@@ -400,7 +400,6 @@ class SurrogateRunner:
         a.flags = _ptr(self.flags_chk)
         a.dt = float(np.float32(sim.p.dt * self.K))
         a.physical_dt = sim.p.dt * self.K
-        a.run_if = _ptr(self.ctl)
         return a
 
     def _issue(self, s: int) -> None:
@@ -420,7 +419,7 @@ class SurrogateRunner:
         infer_cells(self.net, sim, self.ws, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp,
                     go=self.ctl)
         a = self._check_args(s)
-        abi.check(lib.pfh2d_launch_step(C.byref(a), sim.grid, sp, None, 0, 0.0, 0), "check step")
+        abi.check(lib.pfh2d_launch_step_if(C.byref(a), go, sim.grid, sp), "check step")
         abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
                                     self.tol, _ptr(self.err), rej, _ptr(self.counters), go, sp),

@@ -40,7 +40,7 @@ def test_cell_state_layout_matches_reference():
     assert d.fields["reserved"][1] == 120
     assert d.fields["M_pf"][1] == 188
     assert abi.TRAJ_DTYPE.itemsize == 48
-    assert C.sizeof(abi.StepArgs) == 224
+    assert C.sizeof(abi.StepArgs) == 216
 
 
 def test_invalid_arguments_rejected_without_gpu():

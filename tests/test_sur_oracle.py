@@ -1,7 +1,8 @@
 """Pins the surrogate network's CPU oracle (oracle/sur_ref.py): its
 backward pass against central finite differences in float64, the
-bfloat16 rounding rule against known bit patterns, and the backward-data
-weight flip against the direct adjoint definition."""
+bfloat16 rounding rule against known bit patterns, the backward-data
+weight flip against the direct adjoint definition, and the convolution's
+tap offsets, channel order and zero padding against single impulses."""
 import numpy as np
 import pytest
 
@@ -70,3 +71,25 @@ def test_dgrad_is_the_adjoint_of_conv():
         for t in range(9):
             Wt[ci, t, :] = W[:, 8 - t, ci]
     assert np.allclose(R.conv_dgrad(Y, W), R.conv(Y, Wt))
+
+
+def test_conv_impulse_fixes_tap_offsets_and_padding():
+    """A single input impulse through a single non-zero weight lands at the
+    pixel the definition out[p] = sum_t X[p + (ky-1, kx-1)] W[t], t = 3 ky + kx,
+    names, in the named output channel, and nothing else; an impulse on the
+    crop edge loses exactly the taps that would read outside (zero padding)."""
+    from oracle import sur_ref as R
+    H = 7
+    for (y0, x0) in ((3, 3), (0, 0), (6, 2)):
+        for t in range(9):
+            ky, kx = divmod(t, 3)
+            X = np.zeros((1, H, H, 4))
+            X[0, y0, x0, 2] = 1.0
+            W = np.zeros((5, 9, 4))
+            W[3, t, 2] = 0.75
+            out = R.conv(X, W)
+            py, px = y0 - (ky - 1), x0 - (kx - 1)
+            expect = np.zeros_like(out)
+            if 0 <= py < H and 0 <= px < H:
+                expect[0, py, px, 3] = 0.75
+            assert np.array_equal(out, expect), (y0, x0, t)
