@@ -16,6 +16,7 @@
 #define PFO_NUM_CLASSES 6
 #define PFO_CLASS_FALLBACK 5
 #define PFO_CLASS_STAGED_MAX 3
+#define PFO_CONTROL_LANES 16   /* MI355X order: threads 0-15 own no pixels */
 #define PFO_PROMOTE_SLACK 8
 #define PFO_DEMOTE_SLACK 20
 #define PFO_DEMOTE_DWELL 200
@@ -65,11 +66,12 @@ static pfo_layout pfo_layout_of(int order) {
 }
 
 /* Thread owning source pixel (x,y) in the interaction and stencil passes.
- * MI355X staged classes: bands of rows, one column per thread
- * (NB = 1024/WX bands of RB = ceil(WY/NB) rows); non-staged classes: flat
- * pixel index mod 1024. Reference: strips of 16 rows with a 768 stride
- * (src/kernels.cu:265) for classes 0-4; warp rows and lane columns for the
- * fallback (:721-724). */
+ * MI355X staged classes: bands of rows, one column per pixel thread; the
+ * pixel threads are threads 16..1023 (threads 0-15 fetch the next cell):
+ * NB = 1008/WX bands of RB = ceil(WY/NB) rows, thread 16 + band*WX + x;
+ * non-staged classes: flat pixel index mod 1024. Reference: strips of 16
+ * rows with a 768 stride (src/kernels.cu:265) for classes 0-4; warp rows and
+ * lane columns for the fallback (:721-724). */
 static inline int pfo_thr_src(const pfo_layout* l, int CLS, int WX, int WY,
                               int x, int y) {
     if (l->order == PFO_ORDER_REFERENCE) {
@@ -77,9 +79,9 @@ static inline int pfo_thr_src(const pfo_layout* l, int CLS, int WX, int WY,
         return (y % 24) * 32 + (x % 32);
     }
     if (CLS <= PFO_CLASS_STAGED_MAX) {
-        const int nb = 1024 / WX;
+        const int nb = (1024 - PFO_CONTROL_LANES) / WX;
         const int rb = (WY + nb - 1) / nb;
-        return (y / rb) * WX + x;
+        return PFO_CONTROL_LANES + (y / rb) * WX + x;
     }
     return (y * WX + x) % 1024;
 }

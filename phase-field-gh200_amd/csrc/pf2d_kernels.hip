@@ -60,11 +60,14 @@ struct Smem {
     alignas(16) float phi[kPhiLdsFloats];
 };
 
-// Control lanes: the last 16 lanes of the workgroup. No LDS-path class gives
-// them pixels (7x144, 5x176, 7x144 and 6x160 threads are at most 1008), so
-// they fetch the next cell while the other lanes compute.
-constexpr int kControlTid0 = kNT - 16;
-constexpr int kLeaderLane = kControlTid0 % kWave;     // lane 48 of wave 15
+// Control lanes: the first 16 lanes of the workgroup (kControlLanes). No
+// LDS-path class gives them pixels (7x144, 5x176, 7x144 and 6x160 pixel
+// threads are at most 1008), so they fetch the next cell while the other
+// lanes compute. They sit in wave 0, which the SIMD issues first and which
+// therefore finishes its pixels first: the fetch's serial latency (the next
+// cell's scalars) then stays off the barrier's critical path, where it lay
+// in the last wave.
+constexpr int kLeaderLane = 0;                        // lane 0 of wave 0
 
 // threadIdx.x made opaque inside the persistent loop: without this, LICM
 // hoists every per-thread index and address of the cell body out of the loop
@@ -456,10 +459,11 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     constexpr int RB = G::RB, NCH = G::NCH, NCHUNK = G::NCHUNK, KCH = G::KCH;
     constexpr bool kExact = G::NB * RB == WY;   // no partial band
     const int tid = opaque_tid();
-    const int x = tid % WX;
-    const int band = tid / WX;
-    const bool active = band < G::NB;
-    const bool control = tid >= kControlTid0;
+    const int pix = tid - kControlLanes;                 // pixel thread index
+    const bool control = pix < 0;
+    const int x = control ? 0 : pix % WX;
+    const int band = control ? 0 : pix / WX;
+    const bool active = !control && band < G::NB;
     const int y0 = band * RB;
     Fetch nf;
     if (control) fetch1(A, nf);
@@ -484,16 +488,27 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 tin + (size_t)(TY0 + r) * kTilePitch + TX0 + 4 * c4);
         }
     }
-    // ... and this thread's column of the S window.
+    // ... and this thread's column of the S window. S is addressed by a
+    // 32-bit byte offset (4 P L < 2^32, checked at launch) that advances one
+    // row pitch per row and wraps once at the domain edge.
+    const uint32_t s_pitch = 4u * (uint32_t)P, s_wrap = s_pitch * (uint32_t)L;
+    uint32_t s_off0;
+    {
+        int gy = gy0 + y0;
+        if (gy >= L) gy -= L;
+        s_off0 = (uint32_t)gy * s_pitch + 4u * (uint32_t)gx;
+    }
     auto load_qs = [&](uint32_t (&qS)[RB]) {
+        uint32_t off = s_off0;
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             qS[k] = 0u;
-            if (active && row_ok(k)) {
-                int gy = gy0 + y0 + k;
-                if (gy >= L) gy -= L;
-                qS[k] = S_rd[(size_t)gy * P + gx];
-            }
+            if (active && row_ok(k))
+                qS[k] = *reinterpret_cast<const PFH_GLOBAL uint32_t*>(
+                    reinterpret_cast<const PFH_GLOBAL char*>(S_rd) + off);
+            off += s_pitch;
+            if (off >= s_wrap) off -= s_wrap;
+            asm volatile("" : "+v"(off));   // one live offset, not one per row
         }
     };
     uint32_t qS[RB];
@@ -582,8 +597,11 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         float cW, cC, cE;
         load_row(col + (y0 + 1) * PX, cW, cC, cE);
         PFH_GLOBAL float* orow = tout + (ptrdiff_t)(dty0 + y0 - sy) * kTilePitch + dtx0 + a;
-        int gy = gy0 + y0;
-        if (gy >= L) gy -= L;
+        // the aggregate's base in a VGPR pair: the per-row atomic then needs
+        // neither the argument block nor a scalar wait
+        PFH_GLOBAL char* S_sc = reinterpret_cast<PFH_GLOBAL char*>(gptr(A.S_sc));
+        asm volatile("" : "+v"(S_sc));
+        uint32_t s_off = s_off0;
         const double ad = (double)a;
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
@@ -606,7 +624,10 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 clampf |= (sq > kQClampPhiSq);
                 pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pv)));
                 const uint32_t qv = __float2uint_rn(fminf(sq, kQClampPhiSq) * kQScale);
-                if (qv) olds[k] = atomic_add_g(&A.S_sc[(size_t)gy * P + gx], qv);
+                if (qv)
+                    olds[k] = __hip_atomic_fetch_add(
+                        reinterpret_cast<PFH_GLOBAL uint32_t*>(S_sc + s_off), qv,
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const double d = (double)pv * (double)pv;
                 aV += d;
                 aCy += d * (double)b;
@@ -614,7 +635,9 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 sC = cC;
                 cW = nW; cC = nC; cE = nE;
                 orow += kTilePitch;
-                if (++gy == L) gy = 0;
+                s_off += s_pitch;
+                if (s_off >= s_wrap) s_off -= s_wrap;
+                asm volatile("" : "+v"(s_off));
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -731,7 +754,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
     constexpr int TX0 = kShapes[CLS].tx0, TY0 = kShapes[CLS].ty0;
     const int tid = opaque_tid();
     const int L = A.L, P = A.P;
-    if (tid >= kControlTid0) fetch_all(A, sm, nxt, step);
+    if (tid < kControlLanes) fetch_all(A, sm, nxt, step);
     const int gx0 = c.gx0, gy0 = c.gy0;
     const float* tin = A.phi_in + (size_t)n * kTileArea;
     float* tout = A.phi_out + (size_t)n * kTileArea;
@@ -888,7 +911,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     // The control wavefront fetches each next cell one cell ahead (ticket,
     // id, class, CellState, scalars) while the other wavefronts compute, so
     // the queue's global round trips stay off the critical path.
-    if (tid >= kControlTid0) fetch_all(A, sm, 0, step);
+    if (tid < kControlLanes) fetch_all(A, sm, 0, step);
     __syncthreads();
     int cur = 0;
     for (;;) {
@@ -903,7 +926,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
             case 4: cell_cold<4>(&sm.args, n, step, &sm, cur); break;
             case 5: cell_cold<5>(&sm.args, n, step, &sm, cur); break;
             default:                       // unsupported id, flagged at fetch
-                if (tid >= kControlTid0) fetch_all(A, sm, cur ^ 1, step);
+                if (tid < kControlLanes) fetch_all(A, sm, cur ^ 1, step);
                 __syncthreads();
                 break;
         }
@@ -1194,6 +1217,9 @@ int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* str
     // Every window must fit strictly inside the periodic domain
     // (include/params.cuh:666-680), or S self-subtraction is not exact.
     if (a->L <= 286) return invalid("domain side must exceed 286");
+    // the step addresses S with 32-bit byte offsets
+    if (4ull * (unsigned long long)a->P * (unsigned long long)a->L > 0xFFFFFFFFull)
+        return invalid("aggregate larger than 4 GiB");
     hipLaunchKernelGGL(pfh::k_step, dim3((unsigned)grid), dim3(pfh::kNT), 0,
                        as_stream(stream), *a, run_if);
     return launch_status("k_step");

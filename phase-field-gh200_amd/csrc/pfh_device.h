@@ -30,7 +30,8 @@ constexpr int kNT = 1024;
 constexpr int kWave = 64;
 constexpr int kWaves = kNT / kWave;
 // Threads that may own pixels in the LDS path (column bands).
-constexpr int kPixThreads = kNT;
+constexpr int kControlLanes = 16;   // lanes 0-15 of wave 0: next-cell fetch
+constexpr int kPixThreads = kNT - kControlLanes;
 
 // ---- fixed tile and shape classes (include/params.cuh:121-147) ----
 constexpr int kTilePitch = 288;
@@ -237,8 +238,9 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // ---- moments of one cell's window with the update's thread mapping ----
-// Column bands of threads (x = tid % wx, band = tid / wx, contiguous rows per
-// band) for the LDS classes, flat pixel index for classes 4-5; per-thread
+// Column bands of pixel threads (pix = tid - kControlLanes, x = pix % wx,
+// band = pix / wx, contiguous rows per band) for the LDS classes, flat pixel
+// index for classes 4-5; per-thread
 // fp64 sums in visiting order, the lane tree, then waves ascending from 0.
 struct WindowMoments {
     double V, Cx, Cy;
@@ -271,8 +273,9 @@ __device__ WindowMoments window_moments(int cls, F phi_at) {
     };
     if (cls <= kClassStagedMax) {
         const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
-        const int x = tid % wx, band = tid / wx;
-        if (band < nb)
+        const int pix = tid - kControlLanes;
+        const int x = pix % wx, band = pix / wx;
+        if (pix >= 0 && band < nb)
             for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);
     } else {
         for (int p = tid; p < wx * wy; p += kNT) visit(p % wx, p / wx);
