@@ -337,17 +337,29 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
     int lo0 = B, lo1 = B, lo2 = B, hi0 = -1, hi1 = -1, hi2 = -1;
     uint32_t pmax = 0u;
     const int wy = wrap_off(oy, y, L);
+    // S words of this lane's 4 voxels in plane z, fetched one plane ahead (used
+    // only where phi != 0, as before)
+    auto load_s = [&](int z, uint32_t (&sv)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sv[j] = 0u;
+        if (act && z < ze) {
+            const size_t srow = s_index(L, P, 0, wy, wrap_off(oz, z, L));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sv[j] = Sin[srow + (size_t)wrap_off(ox, x0 + j, L)];
+        }
+    };
+    uint32_t Sc[4], Sn[4];
+    load_s(zb, Sc);
 
     for (int z = zb; z < ze; ++z) {
         const int cur = (z - zb) & 1;
         const f32x4 An = (z + 2 <= ze) ? load_plane(z + 2) : f32x4{0.f, 0.f, 0.f, 0.f};
+        load_s(z + 1, Sn);
         if (act) {
             const float* pl = lds + cur * PL;
             const f32x4 ym = *reinterpret_cast<const f32x4*>(pl + own - PX);
             const f32x4 yp = *reinterpret_cast<const f32x4*>(pl + own + PX);
             const float xl = pl[own - 1], xr = pl[own + 4];
-            const int wz = wrap_off(oz, z, L);
-            const size_t srow = s_index(L, P, 0, wy, wz);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float c = A0[j];
@@ -363,8 +375,7 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
                 if (c == 0.0f && gx == 0.0f && gy == 0.0f && gz == 0.0f) continue;
                 float other = 0.0f;
                 const int x = x0 + j;
-                if (c != 0.0f)
-                    other = other_field(Sin[srow + (size_t)wrap_off(ox, x, L)], c, cs, a.flags);
+                if (c != 0.0f) other = other_field(Sc[j], c, cs, a.flags);
                 const double sq = (double)c * (double)c;
                 aV += sq;
                 aCx += sq * (double)x;
@@ -390,6 +401,8 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
         Am = A0;
         A0 = Ap;
         Ap = An;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Sc[j] = Sn[j];
     }
 
     // 64-lane trees, then waves in ascending order
@@ -638,11 +651,29 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     const bool row_ok = act && (unsigned)syv < (unsigned)B;
     const int wyv = row_ok ? wrap_off(oy, syv, L) : 0;
     const PFH_GLOBAL uint32_t* Sin = gptr(a.S_in);
+    // S words of iteration i's destination row (source centre plane pbeg + i - 1),
+    // fetched one iteration ahead; indices exactly those the update reads
+    auto load_su = [&](int i, uint32_t (&sv)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sv[j] = 0u;
+        const int zc = pbeg + i - 1;
+        if (i >= 2 && i < pcount && row_ok && (unsigned)zc < (unsigned)B) {
+            const size_t srow = s_index(L, P, 0, wyv, wrap_off(oz, zc, L));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xs = gx0 + j + sx;
+                if ((unsigned)xs < (unsigned)B) sv[j] = Sin[srow + (size_t)wrap_off(ox, xs, L)];
+            }
+        }
+    };
+    uint32_t Su[4], Sun[4];
+    load_su(0, Su);
 
     for (int i = 0; i < pcount; ++i) {
         const int p = pbeg + i;
         const int cur = i & 1;
         const f32x4 Rnn = (i + 2 < pcount) ? load_plane(p + 2) : f32x4{0.f, 0.f, 0.f, 0.f};
+        load_su(i + 1, Sun);
         float A[4] = {0.f, 0.f, 0.f, 0.f}, Q1[4] = {0.f, 0.f, 0.f, 0.f},
               Q0[4] = {0.f, 0.f, 0.f, 0.f}, GX[4] = {0.f, 0.f, 0.f, 0.f},
               GY[4] = {0.f, 0.f, 0.f, 0.f};
@@ -685,7 +716,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                     if (c == 0.0f && lap == 0.0f && gx == 0.0f && gy == 0.0f && gz == 0.0f)
                         continue;
                     const size_t si = srow + (size_t)wrap_off(ox, xs, L);
-                    const float other = other_field(Sin[si], c, cs, a.flags);
+                    const float other = other_field(Su[j], c, cs, a.flags);
                     const float nx = rhs_next(kc, c, lap, gx, gy, gz, other);
                     if (!isfinite(nx)) {
                         raise3(cs, a.flags, PFH3D_FLAG_NONFINITE);
@@ -709,6 +740,8 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
         if (i + 1 < pcount && act) *reinterpret_cast<f32x4*>(lds + (cur ^ 1) * PLU + stage) = Rn;
         __syncthreads();
         Rn = Rnn;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Su[j] = Sun[j];
     }
 
     if ((sx != 0 || sy != 0 || sz != 0) && act) {
