@@ -397,7 +397,7 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
             }
         }
         if (z + 1 < ze && act) *reinterpret_cast<f32x4*>(lds + (cur ^ 1) * PL + own) = Ap;
-        __syncthreads();
+        pfh::lds_barrier();   // the plane buffers are the only shared data here
         Am = A0;
         A0 = Ap;
         Ap = An;
@@ -668,6 +668,17 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     };
     uint32_t Su[4], Sun[4];
     load_su(0, Su);
+    // S_OVERFLOW test of the previous plane's scatters, taken one plane later
+    // so that no atomic's return is waited for in the plane that issued it
+    uint32_t pold[4] = {0u, 0u, 0u, 0u}, pq[4] = {0u, 0u, 0u, 0u};
+    auto check_overflow = [&]() {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (pq[j] != 0u && pold[j] > 0xffffffffu - pq[j])
+                raise3(cs, a.flags, PFH3D_FLAG_S_OVERFLOW);
+            pq[j] = 0u;
+        }
+    };
 
     for (int i = 0; i < pcount; ++i) {
         const int p = pbeg + i;
@@ -693,6 +704,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                 A[j] = s.A; Q1[j] = s.q1; Q0[j] = s.q0; GX[j] = s.gx; GY[j] = s.gy;
             }
         }
+        check_overflow();
         if (i >= 2 && act) {
             // destination plane d = p - 1 - sz from source centre plane p - 1
             const int d = p - 1 - sz;
@@ -723,7 +735,15 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                         continue;
                     }
                     out[j] = nx;
-                    scatter_value(a.S_out, si, nx, cs, a.flags);
+                    // scatter_value (src/pf3d/device_common.cuh:193-205) with
+                    // the overflow test deferred
+                    if (nx * nx > pfh::kQClampPhiSq) raise3(cs, a.flags, PFH3D_FLAG_Q_CLAMP);
+                    const uint32_t q = q_of(nx);
+                    if (q != 0u) {
+                        pold[j] = __hip_atomic_fetch_add(gptr(a.S_out + si), q, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                        pq[j] = q;
+                    }
                 }
             }
             *reinterpret_cast<PFH_GLOBAL f32x4*>(gdst + ((size_t)d * B + gy_) * B + gx0) = out;
@@ -738,11 +758,12 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
             gym[j] = GY[j];
         }
         if (i + 1 < pcount && act) *reinterpret_cast<f32x4*>(lds + (cur ^ 1) * PLU + stage) = Rn;
-        __syncthreads();
+        pfh::lds_barrier();   // the plane buffers are the only shared data here
         Rn = Rnn;
 #pragma unroll
         for (int j = 0; j < 4; ++j) Su[j] = Sun[j];
     }
+    check_overflow();
 
     if ((sx != 0 || sy != 0 || sz != 0) && act) {
         // source voxels of this slab's planes whose destination leaves the brick

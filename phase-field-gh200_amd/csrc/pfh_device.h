@@ -220,6 +220,17 @@ __host__ __device__ inline double tanh_d(double y) {
     return copysign(t, y);
 }
 
+// ---- workgroup barrier that orders LDS only ----
+// __syncthreads() is a release/acquire fence over every address space, so
+// each wave first waits for all its outstanding global loads, stores and
+// atomics. Loops whose barrier only protects LDS buffers use this instead:
+// global prefetches and scatters stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---- wavefront reductions: lane tree with offsets 32..1 ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

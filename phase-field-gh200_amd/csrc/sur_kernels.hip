@@ -714,7 +714,8 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
                 if (half == 0) pr[8 * kHaloCols] = pacc[4];
             }
             preg = pnext;
-            __syncthreads();
+            // the rings are the only data shared between lanes here
+            pfh::lds_barrier();
         }
     }
 }
