@@ -85,6 +85,10 @@ __device__ __forceinline__ void flag_or(uint32_t* flags, int i) {
     atomicOr(&flags[i], 1u);
 }
 
+// Barriers on the staged path order LDS only (lds_barrier): every value one
+// thread passes to another there goes through LDS, and each thread's global
+// stores and scatters are read back, if at all, by the same thread.
+
 // Global (address space 1) views of the hot buffers: pointers read out of the
 // argument struct are otherwise generic, and flat instructions would tie every
 // LDS wait to outstanding memory operations.
@@ -244,7 +248,7 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
         sm.red[wave * 8 + 0] = aIx;
         sm.red[wave * 8 + 1] = aIy;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double sIx = 0.0, sIy = 0.0;
         for (int w = 0; w < kWaves; ++w) {
@@ -263,7 +267,7 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
         cs->Ix = sIx;
         cs->Iy = sIy;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Moments/support reduction and CellState commit (src/kernels.cu:542-591).
@@ -290,7 +294,7 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
         int* iw = sm.ired + wave * 8;
         iw[0] = blox; iw[1] = bhix; iw[2] = bloy; iw[3] = bhiy; iw[4] = pmaxb;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double sV = 0.0, sCx = 0.0, sCy = 0.0, sPer = 0.0;
         int Blox = c.dwx, Bhix = -1, Bloy = c.dwy, Bhiy = -1, Pmax = 0;
@@ -329,7 +333,7 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
                 ++cs->reserved[1];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // -------------------------------------------------------------------------
@@ -529,7 +533,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             phi_s[(j < WX ? 0 : (WY + 1) * PX - WX) + 4 + j] = 0.0f;
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (control) fetch2(A, nf);
 
     const float gam = c.gam, dwC = c.dwC, volC = c.volC;
@@ -712,13 +716,13 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        __syncthreads();                         // all reads of phi_s done
+        lds_barrier();                         // all reads of phi_s done
         if (active) {
 #pragma unroll
             for (int k = 0; k < RB; ++k)
                 if (row_ok(k)) phi_s[(y0 + k + 1) * PX + 4 + x] = pns[k];
         }
-        __syncthreads();
+        lds_barrier();
         for (int i = tid; i < dwx * dwy; i += kNT) {
             const int b = i / dwx, aa = i - (i / dwx) * dwx;
             const int ii = aa + sx, jj = b + sy;
@@ -912,7 +916,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     // id, class, CellState, scalars) while the other wavefronts compute, so
     // the queue's global round trips stay off the critical path.
     if (tid < kControlLanes) fetch_all(A, sm, 0, step);
-    __syncthreads();
+    lds_barrier();
     int cur = 0;
     for (;;) {
         const int cls = __builtin_amdgcn_readfirstlane(sm.slot[cur].cls);
@@ -927,7 +931,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
             case 5: cell_cold<5>(&sm.args, n, step, &sm, cur); break;
             default:                       // unsupported id, flagged at fetch
                 if (tid < kControlLanes) fetch_all(A, sm, cur ^ 1, step);
-                __syncthreads();
+                lds_barrier();
                 break;
         }
         cur ^= 1;
