@@ -37,6 +37,7 @@ for _p in (REPO, REPO / "phase-field-gh200_amd"):
 HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
 BF16_PEAK_TFLOPS = 2500.0       # MI355X published dense BF16 MFMA rate
 TRAFFIC_SUMMARY = REPO / "profiles" / "pf2d_step_L2048_N1024_traffic.md"
+TRAFFIC_SUMMARY_3D = REPO / "profiles" / "pf3d_step_L256_N128_B56_traffic.md"
 
 
 def q_nonzero_count(sim) -> int:
@@ -61,12 +62,13 @@ def window_area(sim) -> int:
     return int(sum(abi.CLASSES[int(c)][0] * abi.CLASSES[int(c)][1] for c in cls))
 
 
-def read_traffic() -> float | None:
-    """HBM bytes per step launch measured with rocprofv3 counters (FETCH_SIZE
-    + WRITE_SIZE, calibrated on a copy kernel), when a summary is committed."""
-    if not TRAFFIC_SUMMARY.exists():
+def read_traffic(path: Path = TRAFFIC_SUMMARY, key: str = "traffic_bytes_per_launch"
+                 ) -> float | None:
+    """HBM bytes per step measured with rocprofv3 counters (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel), when a summary is committed."""
+    if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text())
+    m = re.search(key + r":\s*([0-9.eE+]+)", path.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -166,6 +168,7 @@ def bench_3d(steps: int, warmup: int, with_cpu: bool) -> dict:
                    "graph_steps": sim3d.GRAPH_BODY},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(TRAFFIC_SUMMARY_3D, "traffic_bytes_per_step"),
                      "algorithmic_bytes_per_step": bytes_per_step,
                      "q_nonzero_per_step": Qbar},
     }
