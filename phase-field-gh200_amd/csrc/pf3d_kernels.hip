@@ -983,8 +983,11 @@ int pfh3d_update_slabs(int N, int B, int device) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
         "hipDeviceGetAttribute(multiProcessorCount)");
     if (st != PFH_OK) return st;
-    // about three resident workgroups per compute unit, slabs of >= 4 planes
-    int k = (3 * (cus > 0 ? cus : 1) + N - 1) / N;
+    // one full round of two resident workgroups (of up to 1024 threads) per
+    // compute unit: a partial second round leaves half the chip idle (at
+    // 128 cells: 4 slabs 216 us per step, 6 slabs 228, 8 slabs 229);
+    // slabs of >= 4 planes
+    int k = (2 * (cus > 0 ? cus : 1)) / N;
     if (k < 1) k = 1;
     if (k > B / 4) k = B / 4;
     return k;
