@@ -258,49 +258,39 @@ struct WindowMoments {
     int blox, bhix, bloy, bhiy, pmaxb;
 };
 
-// phi_at(a, b) returns phi at window pixel (a, b); every pixel is visited
-// exactly once, so phi_at may also store or scatter. Workgroup of kNT
-// threads; the result is valid in thread 0.
-template <class F>
-__device__ WindowMoments window_moments(int cls, F phi_at) {
+// Per-thread partial moments and their workgroup reduction (the lane tree,
+// then waves ascending from 0.0); the result is valid in thread 0.
+struct MomentPart {
+    double V, Cx, Cy;
+    int blox, bhix, bloy, bhiy, pmaxb;
+};
+
+__device__ __forceinline__ void moment_visit(MomentPart& m, float pn, int a, int b) {
+    const double d = (double)pn * (double)pn;
+    m.V += d;
+    m.Cx += d * (double)a;
+    m.Cy += d * (double)b;
+    m.pmaxb = max(m.pmaxb, (int)__float_as_uint(fabsf(pn)));
+    if (pn > kSupportEps) {
+        m.blox = min(m.blox, a); m.bhix = max(m.bhix, a);
+        m.bloy = min(m.bloy, b); m.bhiy = max(m.bhiy, b);
+    }
+}
+
+__device__ inline WindowMoments reduce_moments(MomentPart p, int wx, int wy) {
     __shared__ double red[kWaves * 3];
     __shared__ int ired[kWaves * 5];
-    const Shape sc = shape_of(cls);
-    const int wx = sc.wx, wy = sc.wy;
     const int tid = (int)threadIdx.x;
-    double aV = 0.0, aCx = 0.0, aCy = 0.0;
-    int blox = wx, bhix = -1, bloy = wy, bhiy = -1, pmaxb = 0;
-    auto visit = [&](int a, int b) {
-        const float pn = phi_at(a, b);
-        const double d = (double)pn * (double)pn;
-        aV += d;
-        aCx += d * (double)a;
-        aCy += d * (double)b;
-        pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
-        if (pn > kSupportEps) {
-            blox = min(blox, a); bhix = max(bhix, a);
-            bloy = min(bloy, b); bhiy = max(bhiy, b);
-        }
-    };
-    if (cls <= kClassStagedMax) {
-        const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
-        const int pix = tid - kControlLanes;
-        const int x = pix % wx, band = pix / wx;
-        if (pix >= 0 && band < nb)
-            for (int y = band * rb; y < min(band * rb + rb, wy); ++y) visit(x, y);
-    } else {
-        for (int p = tid; p < wx * wy; p += kNT) visit(p % wx, p / wx);
-    }
     const int lane = tid & (kWave - 1), wave = tid / kWave;
-    aV = wave_sum(aV); aCx = wave_sum(aCx); aCy = wave_sum(aCy);
-    blox = wave_min(blox); bhix = wave_max(bhix);
-    bloy = wave_min(bloy); bhiy = wave_max(bhiy);
-    pmaxb = wave_max(pmaxb);
+    p.V = wave_sum(p.V); p.Cx = wave_sum(p.Cx); p.Cy = wave_sum(p.Cy);
+    p.blox = wave_min(p.blox); p.bhix = wave_max(p.bhix);
+    p.bloy = wave_min(p.bloy); p.bhiy = wave_max(p.bhiy);
+    p.pmaxb = wave_max(p.pmaxb);
     if (lane == 0) {
-        red[wave * 3 + 0] = aV; red[wave * 3 + 1] = aCx; red[wave * 3 + 2] = aCy;
-        ired[wave * 5 + 0] = blox; ired[wave * 5 + 1] = bhix;
-        ired[wave * 5 + 2] = bloy; ired[wave * 5 + 3] = bhiy;
-        ired[wave * 5 + 4] = pmaxb;
+        red[wave * 3 + 0] = p.V; red[wave * 3 + 1] = p.Cx; red[wave * 3 + 2] = p.Cy;
+        ired[wave * 5 + 0] = p.blox; ired[wave * 5 + 1] = p.bhix;
+        ired[wave * 5 + 2] = p.bloy; ired[wave * 5 + 3] = p.bhiy;
+        ired[wave * 5 + 4] = p.pmaxb;
     }
     __syncthreads();
     WindowMoments m = {0.0, 0.0, 0.0, wx, -1, wy, -1, 0};
@@ -313,6 +303,28 @@ __device__ WindowMoments window_moments(int cls, F phi_at) {
         }
     }
     return m;
+}
+
+// phi_at(a, b) returns phi at window pixel (a, b); every pixel is visited
+// exactly once, so phi_at may also store or scatter. Workgroup of kNT
+// threads; the result is valid in thread 0.
+template <class F>
+__device__ WindowMoments window_moments(int cls, F phi_at) {
+    const Shape sc = shape_of(cls);
+    const int wx = sc.wx, wy = sc.wy;
+    const int tid = (int)threadIdx.x;
+    MomentPart p = {0.0, 0.0, 0.0, wx, -1, wy, -1, 0};
+    if (cls <= kClassStagedMax) {
+        const int nb = kPixThreads / wx, rb = (wy + nb - 1) / nb;
+        const int pix = tid - kControlLanes;
+        const int x = pix % wx, band = pix / wx;
+        if (pix >= 0 && band < nb)
+            for (int y = band * rb; y < min(band * rb + rb, wy); ++y)
+                moment_visit(p, phi_at(x, y), x, y);
+    } else {
+        for (int q = tid; q < wx * wy; q += kNT) moment_visit(p, phi_at(q % wx, q / wx), q % wx, q / wx);
+    }
+    return reduce_moments(p, wx, wy);
 }
 
 // Moments into the cell state; the interaction integral restarts at zero.

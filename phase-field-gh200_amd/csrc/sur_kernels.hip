@@ -938,6 +938,8 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
     const float* tile = phi_chk + (size_t)n * pfh::kTileArea;
     const int k = (int)cls[n];
     double e = 0.0, s = 0.0;
+    // unrolled so that several pixels' loads are in flight (sums in order)
+#pragma unroll 6
     for (int p = (int)threadIdx.x; p < kCropPix; p += 256) {
         const int i = p % kCrop, j = p / kCrop;
         int wx = ox + i;
@@ -972,6 +974,55 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
 // moments, visiting the window with the update's thread mapping so that V,
 // Cx, Cy equal k_init_moments' sums bit for bit (k_scatter_all /
 // k_init_moments semantics, src/kernels.cu:1098-1190).
+// Staged classes: each pixel thread walks its rows of one column (the update's
+// mapping), keeps the atomics' previous values in registers and tests them for
+// S_OVERFLOW after the loop, so no row waits for its scatter.
+template <int CLS>
+__device__ void commit_staged(float* t, const float* pr, int L, int P, uint32_t* S, int gx0,
+                              int gy0, int ox, int oy, pfh::MomentPart& mp, bool& ovf,
+                              bool& clampf) {
+    constexpr int WX = pfh::kShapes[CLS].wx, WY = pfh::kShapes[CLS].wy;
+    constexpr int NB = pfh::kPixThreads / WX, RB = (WY + NB - 1) / NB;
+    const int pix = (int)threadIdx.x - pfh::kControlLanes;
+    if (pix < 0 || pix / WX >= NB) return;
+    const int x = pix % WX, y0 = (pix / WX) * RB;
+    const int wx = pfh::wrapi(gx0 + x, L);
+    int i = wx - ox;
+    if (i < 0) i += L;
+    const bool icrop = i < kCrop;
+    // rows in chunks of 8: a chunk's scatters are in flight together and their
+    // previous values are tested at the chunk's end (every addend is at most
+    // 2^29, so a wrap needs old >= 2^32 - 2^29; only such candidates
+    // recompute q from the value this thread stored)
+    constexpr int CH = 8;
+    for (int k0 = 0; k0 < RB; k0 += CH) {
+        uint32_t olds[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            olds[k] = 0u;
+            const int y = y0 + k0 + k;
+            if (k0 + k < RB && y < WY) {
+                const int wy = pfh::wrapi(gy0 + y, L);
+                int j = wy - oy;
+                if (j < 0) j += L;
+                const float v = (icrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
+                t[(size_t)y * pfh::kTilePitch + x] = v;
+                const uint32_t q = pfh::q_of(v);
+                if (q) {
+                    olds[k] = atomicAdd(&S[(size_t)wy * P + wx], q);
+                    clampf |= v * v > pfh::kQClampPhiSq;
+                }
+                pfh::moment_visit(mp, v, x, y);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+            if (olds[k] >= 0xE0000000u)
+                ovf |= olds[k] >
+                       0xFFFFFFFFu - pfh::q_of(t[(size_t)(y0 + k0 + k) * pfh::kTilePitch + x]);
+    }
+}
+
 __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_state* cells,
                                                          const uint8_t* cls, const int* origin,
                                                          const float* pred, int L, int P,
@@ -986,22 +1037,34 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
     float* t = phi + (size_t)n * pfh::kTileArea + (size_t)sc.ty0 * pfh::kTilePitch + sc.tx0;
     const float* pr = pred + (size_t)n * kCropPix;
     bool ovf = false, clampf = false;
-    const pfh::WindowMoments m = pfh::window_moments(k, [&](int a, int b) {
-        const int wx = pfh::wrapi(gx0 + a, L), wy = pfh::wrapi(gy0 + b, L);
-        int i = wx - ox;
-        if (i < 0) i += L;
-        int j = wy - oy;
-        if (j < 0) j += L;
-        const float v = (i < kCrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
-        t[(size_t)b * pfh::kTilePitch + a] = v;
-        const uint32_t q = pfh::q_of(v);
-        if (q) {
-            const uint32_t old = atomicAdd(&S[(size_t)wy * P + wx], q);
-            ovf |= old > 0xFFFFFFFFu - q;
-            clampf |= v * v > pfh::kQClampPhiSq;
+    pfh::WindowMoments m;
+    if (k <= pfh::kClassStagedMax) {
+        pfh::MomentPart mp = {0.0, 0.0, 0.0, sc.wx, -1, sc.wy, -1, 0};
+        switch (k) {
+            case 0: commit_staged<0>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf); break;
+            case 1: commit_staged<1>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf); break;
+            case 2: commit_staged<2>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf); break;
+            default: commit_staged<3>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf); break;
         }
-        return v;
-    });
+        m = pfh::reduce_moments(mp, sc.wx, sc.wy);
+    } else {
+        m = pfh::window_moments(k, [&](int a, int b) {
+            const int wx = pfh::wrapi(gx0 + a, L), wy = pfh::wrapi(gy0 + b, L);
+            int i = wx - ox;
+            if (i < 0) i += L;
+            int j = wy - oy;
+            if (j < 0) j += L;
+            const float v = (i < kCrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
+            t[(size_t)b * pfh::kTilePitch + a] = v;
+            const uint32_t q = pfh::q_of(v);
+            if (q) {
+                const uint32_t old = atomicAdd(&S[(size_t)wy * P + wx], q);
+                ovf |= old > 0xFFFFFFFFu - q;
+                clampf |= v * v > pfh::kQClampPhiSq;
+            }
+            return v;
+        });
+    }
     if (ovf) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
     if (clampf) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
     if (threadIdx.x == 0) pfh::store_moments(&cells[n], m);
