@@ -185,7 +185,10 @@ int pfh2d_step_grid(int device);
  * One persistent work-queue launch advances every cell by one step.
  * l2_base/l2_bytes/l2_hit_ratio are accepted and ignored: gfx950 has no
  * L2 access-policy window. observation must be 0 (Disabled); the velocity
- * observers are outside this build (returns PFH_ERR_UNSUPPORTED). */
+ * observers are outside this build (returns PFH_ERR_UNSUPPORTED). S_sc may
+ * be NULL (no reference counterpart): the step then scatters nothing, for a
+ * caller that needs only the new field and cell states (the surrogate's check
+ * step). */
 int pfh2d_launch_step(const pfh2d_step_args* args, int grid, void* stream,
                       const void* l2_base, size_t l2_bytes,
                       float l2_hit_ratio, int observation);
@@ -508,8 +511,8 @@ int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin
                     const int* rejected, long long* abort_step, uint32_t* rejected_blocks,
                     void* stream);
 /* Start of a block (nothing when *go == 0): copies of the cell states and
- * classes for the check step, its aggregate zeroed, cursors zeroed, both
- * step slots = *step_now, *rejected = 0. */
+ * classes for the check step, S_chk[0..words) zeroed (words may be 0 and
+ * S_chk NULL), cursors zeroed, both step slots = *step_now, *rejected = 0. */
 int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,
                  pfh_cell_state* cells_chk, const uint8_t* cls, uint8_t* cls_chk, int N,
                  uint32_t* S_chk, unsigned long long words, unsigned long long* cur_chk,

@@ -603,6 +603,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         PFH_GLOBAL float* orow = tout + (ptrdiff_t)(dty0 + y0 - sy) * kTilePitch + dtx0 + a;
         // the aggregate's base in a VGPR pair: the per-row atomic then needs
         // neither the argument block nor a scalar wait
+        const bool scatter = A.S_sc != nullptr;   // NULL: a step that scatters nothing
         PFH_GLOBAL char* S_sc = reinterpret_cast<PFH_GLOBAL char*>(gptr(A.S_sc));
         asm volatile("" : "+v"(S_sc));
         uint32_t s_off = s_off0;
@@ -628,7 +629,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 clampf |= (sq > kQClampPhiSq);
                 pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pv)));
                 const uint32_t qv = __float2uint_rn(fminf(sq, kQClampPhiSq) * kQScale);
-                if (qv)
+                if (qv && scatter)
                     olds[k] = __hip_atomic_fetch_add(
                         reinterpret_cast<PFH_GLOBAL uint32_t*>(S_sc + s_off), qv,
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -841,7 +842,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         clampf |= (pn * pn > kQClampPhiSq);
         pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
         const uint32_t q = q_of(pn);
-        if (q) {
+        if (q && A.S_sc) {
             const uint32_t old = atomicAdd(
                 &A.S_sc[(size_t)wrapi(gy0n + b, L) * P + wrapi(gx0n + a, L)], q);
             ovf |= old > 0xFFFFFFFFu - q;
@@ -1213,7 +1214,7 @@ int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* str
     if (!a) return invalid("args");
     if (a->moments) return invalid("moments must be NULL");
     if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
-        !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||
+        !a->phi_out || !a->S_rd || !a->cell || !a->cell_cls ||
         !a->perm || !a->cursor_use || !a->cursor_clear || !a->step_rd ||
         !a->step_wr || !a->flags || (a->clear_ahead_words && !a->S_cl) ||
         a->parity_out < 0 || a->parity_out > 1)

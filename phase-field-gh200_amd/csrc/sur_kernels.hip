@@ -1429,8 +1429,8 @@ int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,
                  pfh_cell_state* cells_chk, const uint8_t* cls, uint8_t* cls_chk, int N,
                  uint32_t* S_chk, unsigned long long words, unsigned long long* cur_chk,
                  long long* step_chk, const long long* step_now, void* stream) {
-    if (!go || !rejected || !cells || !cells_chk || !cls || !cls_chk || N <= 0 || !S_chk ||
-        !cur_chk || !step_chk || !step_now ||
+    if (!go || !rejected || !cells || !cells_chk || !cls || !cls_chk || N <= 0 ||
+        (words && !S_chk) || !cur_chk || !step_chk || !step_now ||
         ((((uintptr_t)cells) | ((uintptr_t)cells_chk) | ((uintptr_t)S_chk)) & 15u))
         return invalid("sur_prep");
     int dev = 0, cus = 0;

@@ -367,7 +367,6 @@ class SurrogateRunner:
         self.ws = Workspace(N, dev, hidden=False)
         self.W4d, self.b4d = deploy_weights(net)
         self.phi_chk = torch.zeros_like(sim.phi[0])
-        self.S_chk = torch.zeros(2 * sim.words, dtype=torch.int32, device=dev)
         self.cell_chk = torch.zeros_like(sim.cell)
         self.cls_chk = torch.zeros_like(sim.cls)
         self.cur_chk = torch.zeros(2, dtype=torch.int64, device=dev)
@@ -388,8 +387,8 @@ class SurrogateRunner:
         a = abi.StepArgs()
         C.memmove(C.addressof(a), C.addressof(sim._args[slot % 6]), C.sizeof(a))
         a.phi_out = _ptr(self.phi_chk)
-        a.S_sc = _ptr(self.S_chk)
-        a.S_cl = _ptr(self.S_chk) + 4 * sim.words
+        a.S_sc = None                  # the check step's aggregate is never read
+        a.S_cl = None
         a.clear_ahead_words = 0
         a.cell = _ptr(self.cell_chk)
         a.cell_cls = _ptr(self.cls_chk)
@@ -414,8 +413,8 @@ class SurrogateRunner:
         now = _ptr(sim.stepc) + 8 * pin
         # propose
         abi.check(lib.pfh_sur_prep(go, rej, _ptr(sim.cell), _ptr(self.cell_chk), _ptr(sim.cls),
-                                   _ptr(self.cls_chk), N, _ptr(self.S_chk), sim.words,
-                                   _ptr(self.cur_chk), _ptr(self.step_chk), now, sp), "prep")
+                                   _ptr(self.cls_chk), N, None, 0, _ptr(self.cur_chk),
+                                   _ptr(self.step_chk), now, sp), "prep")
         infer_cells(self.net, sim, self.ws, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp,
                     go=self.ctl)
         a = self._check_args(s)
