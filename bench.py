@@ -177,16 +177,17 @@ def bench_3d(steps: int, warmup: int, with_cpu: bool) -> dict:
     return out
 
 
-def bench_surrogate(blocks: int) -> dict | None:
+def bench_surrogate(blocks: int, path=None) -> dict | None:
     """Surrogate + fallback against the plain method on the same 2048^2 /
-    1024-cell state (8-step blocks, check rule and tolerance of
-    pfh.surrogate.SurrogateRunner), with the committed weights."""
+    1024-cell state (blocks of the weights' K steps, check rule and
+    tolerance of pfh.surrogate.SurrogateRunner), with committed weights."""
     from pfh import surrogate as su
     from pfh import surrogate_train as st
-    if not su.WEIGHTS_PATH.exists():
-        return {"error": f"no weights at {su.WEIGHTS_PATH.name}"}
-    net = su.Net.load(su.WEIGHTS_PATH)
-    r = st.evaluate(net, blocks=blocks)
+    path = su.WEIGHTS_PATH if path is None else path
+    if not path.exists():
+        return {"error": f"no weights at {path.name}"}
+    net = su.Net.load(path)
+    r = st.evaluate(net, blocks=blocks, K=net.K)
     inf = r["infer"]
     r["roofline"] = {"bound": "mfma", "kernel": inf["kernel"], "achieved": inf["tflops"],
                      "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -305,7 +306,12 @@ def main() -> int:
     del sim
     torch.cuda.empty_cache()
     if args.sur_blocks > 0:
+        from pfh import surrogate as su
         out["surrogate"] = bench_surrogate(args.sur_blocks)
+        torch.cuda.empty_cache()
+        # the 16-step variant over about the same number of steps
+        out["surrogate_k16"] = bench_surrogate(max(1, args.sur_blocks // 2),
+                                               su.WEIGHTS_PATH_K16)
         torch.cuda.empty_cache()
     if args.workload == "all":
         out["pf3d"] = bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)

@@ -40,6 +40,9 @@ SIZES = {"W1": CH * 9 * C0, "b1": CH, "W2": CH * 9 * CH, "b2": CH, "W3": CH * 9 
 ORDER = list(SIZES)
 NPARAM = sum(SIZES.values())
 WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "surrogate_k8.npz"
+# the same network trained for 16-step blocks (a variant beyond the 8-step
+# check cadence; see pfh.surrogate_train)
+WEIGHTS_PATH_K16 = WEIGHTS_PATH.with_name("surrogate_k16.npz")
 
 
 def _ptr(t: torch.Tensor) -> int:
@@ -58,6 +61,7 @@ class Net:
         self.v = torch.zeros_like(self.w)
         self.g = torch.zeros_like(self.w)
         self.scale = float(scale)
+        self.K = 8          # steps per block the weights were trained for
         self.t = 0
         self.views = {}
         self.gviews = {}
@@ -106,12 +110,13 @@ class Net:
     def save(self, path: Path) -> None:
         path.parent.mkdir(parents=True, exist_ok=True)
         h = self.host()
-        np.savez(path, scale=np.float64(self.scale), **h)
+        np.savez(path, scale=np.float64(self.scale), K=np.int64(self.K), **h)
 
     @classmethod
     def load(cls, path: Path = WEIGHTS_PATH, device="cuda:0") -> "Net":
         z = np.load(path, allow_pickle=False)
         net = cls(device=device, scale=float(z["scale"]))
+        net.K = int(z["K"]) if "K" in z.files else 8
         net.set_host({k: z[k] for k in ORDER})
         return net
 

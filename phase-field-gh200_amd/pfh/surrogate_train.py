@@ -33,12 +33,14 @@ def log(msg: str) -> None:
 def cmd_train(a) -> int:
     seeds = list(range(a.seeds[0], a.seeds[1] + 1))
     t0 = time.perf_counter()
-    X, D = su.make_dataset(seeds, samples_per_seed=a.samples, warm=a.warm, gap=a.gap, log=log)
+    X, D = su.make_dataset(seeds, samples_per_seed=a.samples, K=a.K, warm=a.warm, gap=a.gap,
+                           log=log)
     n = D.numel() // su.PIX
     log(f"{n} crops in {time.perf_counter() - t0:.1f} s")
     held = min(2048, n // 10)
     ntr = n - held
     net = su.Net(seed=a.init_seed, scale=a.scale)
+    net.K = a.K
     t0 = time.perf_counter()
     hist = su.train(net, X[:ntr * su.PIX * su.C0], D[:ntr * su.PIX], epochs=a.epochs,
                     batch=a.batch, lr=a.lr, log=log)
@@ -53,7 +55,7 @@ def cmd_train(a) -> int:
         e_net += float(((ws.dphi[:m * su.PIX].double() * net.scale - d) ** 2).sum())
         e_id += float((d ** 2).sum())
     net.save(Path(a.out))
-    rep = {"crops": n, "train_crops": ntr, "held_out": held, "epochs": a.epochs,
+    rep = {"K": a.K, "crops": n, "train_crops": ntr, "held_out": held, "epochs": a.epochs,
            "batch": a.batch, "train_seconds": dt, "loss_history": hist,
            "held_out_increment_rel_error_network": (e_net / e_id) ** 0.5,
            "held_out_increment_rel_error_no_change": 1.0, "weights": a.out}
@@ -151,7 +153,7 @@ def time_infer(runner: su.SurrogateRunner, reps: int = 20) -> dict:
 
 def cmd_eval(a) -> int:
     net = su.Net.load(Path(a.weights))
-    out = evaluate(net, blocks=a.blocks, tol=a.tol)
+    out = evaluate(net, blocks=a.blocks, K=a.K or net.K, tol=a.tol)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -164,6 +166,7 @@ def main(argv=None) -> int:
     t.add_argument("--samples", type=int, default=2)
     t.add_argument("--warm", type=int, default=300)
     t.add_argument("--gap", type=int, default=120)
+    t.add_argument("--K", type=int, default=8, help="steps per surrogate block")
     t.add_argument("--epochs", type=int, default=12)
     t.add_argument("--batch", type=int, default=128)
     t.add_argument("--lr", type=float, default=2e-3)
@@ -174,6 +177,7 @@ def main(argv=None) -> int:
     e.add_argument("--weights", default=str(su.WEIGHTS_PATH))
     e.add_argument("--blocks", type=int, default=25)
     e.add_argument("--tol", type=float, default=1e-3)
+    e.add_argument("--K", type=int, default=0, help="steps per block (default: the weights')")
     a = ap.parse_args(argv)
     return cmd_train(a) if a.cmd == "train" else cmd_eval(a)
 

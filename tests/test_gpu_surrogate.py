@@ -325,6 +325,23 @@ def test_committed_weights_accuracy_and_acceptance():
     assert r["rel_l2_vs_plain_max"] <= 1e-3
 
 
+def test_committed_k16_weights_accuracy_and_acceptance():
+    """The committed 16-step weights: 6 blocks (96 steps) from step 300 on the
+    2048^2 / 1024-cell monolayer, every block accepted, within 1e-3 relative
+    L2 of the plain run."""
+    from pfh import surrogate_train as st
+    assert su.WEIGHTS_PATH_K16.exists(), su.WEIGHTS_PATH_K16
+    net = su.Net.load(su.WEIGHTS_PATH_K16)
+    assert net.K == 16
+    say("committed K=16 weights: 6 blocks on C2 against the plain method")
+    r = st.evaluate(net, blocks=6, K=16)
+    say(f"accepted {r['accepted_blocks']}/{r['blocks']}, rel L2 vs plain max "
+        f"{r['rel_l2_vs_plain_max']:.3e}, speedup {r['speedup']:.3f}")
+    assert not r["fatal"]
+    assert r["fallback_blocks"] == 0
+    assert r["rel_l2_vs_plain_max"] <= 1e-3
+
+
 def _state(sim):
     return [sim.phi[0], sim.phi[1], sim.S, sim.cell, sim.cls, sim.stepc, sim.cursor, sim.flags]
 
