@@ -19,10 +19,15 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: binary32 arithmetic in source order, as the reference's
 # host semantics define it. Packed FP32 instructions are disabled on purpose.
+# The atomic optimizer is off: it turns a single lane's queue-ticket atomic
+# into a wave-wide broadcast that waits for the atomic's return at once (and
+# with it for every load in flight); no kernel here has many lanes adding to
+# one address on a hot path.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-I", str(REPO / "include"), "-I", str(CSRC),
 ]
 

@@ -57,8 +57,12 @@ struct Smem {
     int ired[kWaves * 8];
     Slot slot[2];
     uint32_t csraw[2][48];       // CellState (192 B) of the next cell
-    alignas(16) float phi[kPhiLdsFloats];
 };
+
+// The staged window, an LDS object of its own: the global-to-LDS window
+// copies write only here, so the compiler does not hold other LDS reads
+// (slots, partials) until those copies land.
+static __shared__ __attribute__((aligned(16))) float s_window[kPhiLdsFloats];
 
 // Control lanes: the first 16 lanes of the workgroup (kControlLanes). No
 // LDS-path class gives them pixels (7x144, 5x176, 7x144 and 6x160 pixel
@@ -68,6 +72,7 @@ struct Smem {
 // cell's scalars) then stays off the barrier's critical path, where it lay
 // in the last wave.
 constexpr int kLeaderLane = 0;                        // lane 0 of wave 0
+constexpr int kScalarsThread = kWave;                 // lane 0 of wave 1
 
 // threadIdx.x made opaque inside the persistent loop: without this, LICM
 // hoists every per-thread index and address of the cell body out of the loop
@@ -271,11 +276,26 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
 }
 
 // Moments/support reduction and CellState commit (src/kernels.cu:542-591).
-template <int CLS>
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// after_partials runs on every thread once the wave partials are published
+// (every wave has finished the cell's pixel phases).
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// old_raw: the cell's CellState as fetched at the start of its turn (LDS),
+// for the counters that the commit advances.
+template <int CLS, class Hook = NoHook>
 __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
-                              const Ctl& c, double aV, double aCx, double aCy, double aPer,
+                              const Ctl& c, const uint32_t* old_raw,
+                              double aV, double aCx, double aCy, double aPer,
                               int blox, int bhix, int bloy, int bhiy,
-                              int pmaxb) {
+                              int pmaxb, Hook after_partials = Hook()) {
     const int tid = opaque_tid();
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -295,17 +315,33 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
         iw[0] = blox; iw[1] = bhix; iw[2] = bloy; iw[3] = bhiy; iw[4] = pmaxb;
     }
     lds_barrier();
-    if (threadIdx.x == 0) {
-        double sV = 0.0, sCx = 0.0, sCy = 0.0, sPer = 0.0;
-        int Blox = c.dwx, Bhix = -1, Bloy = c.dwy, Bhiy = -1, Pmax = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            const double* rw = sm.red + w * 8;
-            sV += rw[0]; sCx += rw[1]; sCy += rw[2]; sPer += rw[3];
-            const int* iw = sm.ired + w * 8;
-            Blox = min(Blox, iw[0]); Bhix = max(Bhix, iw[1]);
-            Bloy = min(Bloy, iw[2]); Bhiy = max(Bhiy, iw[3]);
-            Pmax = max(Pmax, iw[4]);
+    after_partials();
+    if (__builtin_amdgcn_readfirstlane(wave) == 0) {
+        // Lanes 0-3 add the 16 wave partials of V, Cx, Cy and the perimeter,
+        // lanes 4-8 reduce the support box and phi_max, each in wave order
+        // from the same initial value: the sums of one thread doing all nine.
+        double sl = 0.0;
+        int ml = 0;
+        if (lane < 4) {
+            for (int w = 0; w < kWaves; ++w) sl += sm.red[w * 8 + lane];
+        } else if (lane < 9) {
+            const int j = lane - 4;                  // 0 lo_x, 1 hi_x, 2 lo_y, 3 hi_y, 4 pmax
+            const bool is_max = (j & 1) != 0 || j == 4;
+            ml = j == 0 ? c.dwx : j == 2 ? c.dwy : j == 4 ? 0 : -1;
+            for (int w = 0; w < kWaves; ++w) {
+                const int v = sm.ired[w * 8 + j];
+                ml = is_max ? max(ml, v) : min(ml, v);
+            }
         }
+        const double sV = readlane_f64(sl, 0), sCx = readlane_f64(sl, 1);
+        const double sCy = readlane_f64(sl, 2), sPer = readlane_f64(sl, 3);
+        const int Blox = __builtin_amdgcn_readlane(ml, 4);
+        const int Bhix = __builtin_amdgcn_readlane(ml, 5);
+        const int Bloy = __builtin_amdgcn_readlane(ml, 6);
+        const int Bhiy = __builtin_amdgcn_readlane(ml, 7);
+        const int Pmax = __builtin_amdgcn_readlane(ml, 8);
+        const pfh_cell_state* old = reinterpret_cast<const pfh_cell_state*>(old_raw);
+        if (lane == 0) {
         pfh_cell_state* cs = &A.cell[n];
         cs->gx0 = c.gx0n;
         cs->gy0 = c.gy0n;
@@ -321,8 +357,8 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
         cs->bb_lo_y = Bloy;
         cs->bb_hi_y = Bhiy;
         cs->promote_ctr = (uint32_t)c.pctr;
-        cs->shift_ctr += (c.sx | c.sy) ? 1u : 0u;
-        cs->tumble_ctr += (uint32_t)c.tumbled;
+        cs->shift_ctr = old->shift_ctr + ((c.sx | c.sy) ? 1u : 0u);
+        cs->tumble_ctr = old->tumble_ctr + (uint32_t)c.tumbled;
         cs->phi_max = __uint_as_float((uint32_t)Pmax);
         A.cell_cls[n] = (uint8_t)c.dcls;
         if (CLS == kClassFallback) {
@@ -330,7 +366,8 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
             const bool edge = Bhix >= 0 && (Blox == 0 || Bhix == c.dwx - 1 ||
                                             Bloy == 0 || Bhiy == c.dwy - 1);
             if (c.no_margin || (c.dcls == kClassFallback && edge))
-                ++cs->reserved[1];
+                cs->reserved[1] = old->reserved[1] + 1u;
+        }
         }
     }
     lds_barrier();
@@ -375,8 +412,9 @@ __device__ __forceinline__ void fetch3(const pfh2d_step_args& A, Fetch& f) {
     }
 }
 
-__device__ __forceinline__ void fetch4(const pfh2d_step_args& A, Smem& sm, Fetch& f,
-                                       int which, unsigned long long step) {
+// 4a (control lanes): the CellState copy, the id and the class into LDS.
+__device__ __forceinline__ void fetch4a(const pfh2d_step_args& A, Smem& sm, Fetch& f,
+                                        int which) {
     const int j = ((int)threadIdx.x & (kWave - 1)) - kLeaderLane;
     int cls = -1;
     if (f.nn >= 0) {
@@ -389,11 +427,23 @@ __device__ __forceinline__ void fetch4(const pfh2d_step_args& A, Smem& sm, Fetch
         if (cls == kNumClasses) atomicAdd(&A.flags[PFH_FLAG_CLASS_UNSUPPORTED], 1u);
         sm.slot[which].n = f.nn;
         sm.slot[which].cls = cls;
-        if (cls >= 0 && cls < kNumClasses)
-            cell_scalars(scalar_args(A),
-                         reinterpret_cast<const pfh_cell_state*>(sm.csraw[which]), cls,
-                         step, &sm.slot[which].ctl);
     }
+}
+
+// 4b (one thread, after 4a is visible to it): the per-cell scalars.
+__device__ __forceinline__ void fetch4b(const pfh2d_step_args& A, Smem& sm, int which,
+                                        unsigned long long step) {
+    const int cls = sm.slot[which].cls;
+    if (cls >= 0 && cls < kNumClasses)
+        cell_scalars(scalar_args(A),
+                     reinterpret_cast<const pfh_cell_state*>(sm.csraw[which]), cls,
+                     step, &sm.slot[which].ctl);
+}
+
+__device__ __forceinline__ void fetch4(const pfh2d_step_args& A, Smem& sm, Fetch& f,
+                                       int which, unsigned long long step) {
+    fetch4a(A, sm, f, which);
+    if (((int)threadIdx.x & (kWave - 1)) == kLeaderLane) fetch4b(A, sm, which, step);
 }
 
 __device__ __forceinline__ void fetch_all(const pfh2d_step_args& A, Smem& sm, int which,
@@ -454,9 +504,43 @@ __device__ __forceinline__ float s_other_of(uint32_t qS, float c, uint32_t& negc
     return neg ? 0.0f : (float)(qS - qc) * kQInv;
 }
 
+// The same, counting the wave's negative differences in a scalar register
+// (the count of the active lanes whose S is below their own q).
+__device__ __forceinline__ float s_other_wave(uint32_t qS, float c, uint32_t& negw) {
+    const uint32_t qc = q_of(c);
+    const bool neg = qS < qc;
+    negw += (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(neg));
+    return neg ? 0.0f : (float)(qS - qc) * kQInv;
+}
+
+// Class-0 window prefetch: once a class-0 cell's pixel phases are done with
+// the LDS window, and the next cell in the queue is also of class 0, the
+// next window's rows are copied from HBM straight into LDS (global-to-LDS
+// loads, no registers) while the current cell's moments are committed. Wave
+// w >= 1 copies rows w - 1, w + 14, ...; one instruction moves one 576 B row
+// with 36 lanes of 16 B into the row's place in the padded layout. Wave 0
+// (thread 0 commits the cell, the control lanes fetch) copies nothing, so
+// that none of its own memory waits has to wait for the copies.
+__device__ __forceinline__ void prefetch_window0(const pfh2d_step_args& A, Smem& sm,
+                                                 int nn) {
+    using G = Staged<0>;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+    const PFH_GLOBAL float* src = gptr(A.phi_in) + (size_t)nn * kTileArea
+                                + (size_t)G::TY0 * kTilePitch + G::TX0 + 4 * lane;
+    if (wave > 0 && lane < G::NCH) {
+        for (int r = wave - 1; r < G::WY; r += kWaves - 1)
+            __builtin_amdgcn_global_load_lds(
+                (const PFH_GLOBAL void*)(src + (size_t)r * kTilePitch),
+                (__attribute__((address_space(3))) void*)(s_window + (r + 1) * G::PX + 4),
+                16, 0, 0);
+    }
+}
+
 template <int CLS>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
-                            unsigned long long step, Smem& sm, Ctl& c, int nxt) {
+                            unsigned long long step, Smem& sm, Ctl& c, int nxt,
+                            int& pre) {
     using G = Staged<CLS>;
     constexpr int WX = G::WX, WY = G::WY, TX0 = G::TX0, TY0 = G::TY0;
     constexpr int PX = G::PX;
@@ -472,24 +556,31 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     Fetch nf;
     if (control) fetch1(A, nf);
     const int L = A.L, P = A.P;
-    const int gx0 = A.cell[n].gx0;
-    const int gy0 = A.cell[n].gy0;
-    float* phi_s = sm.phi;
+    // the origin from the LDS copy of the CellState: a global load here would
+    // make the S loads wait for every load issued before it
+    const int gx0 = c.gx0;
+    const int gy0 = c.gy0;
+    float* phi_s = s_window;
     const float* col = phi_s + 4 + x;           // row r of column x: col[r*PX]
     const int gx = (gx0 + x < L) ? gx0 + x : gx0 + x - L;
-    auto row_ok = [&](int k) { return kExact || y0 + k < WY; };
+    // k is a compile-time row index: rows below RFULL exist in every band, so
+    // only the last band's tail rows are tested at run time.
+    auto row_ok = [&](int k) { return kExact || k < G::RFULL || y0 + k < WY; };
 
     // Window rows of phi in flight (16 B each) ...
     const PFH_GLOBAL float* tin = gptr(A.phi_in) + (size_t)n * kTileArea;
     const PFH_GLOBAL uint32_t* S_rd = gptr(A.S_rd);
+    const bool have = CLS == 0 && pre != 0;     // window already copied into LDS
     f32x4 ph[KCH];
+    if (!have) {
 #pragma unroll
-    for (int k = 0; k < KCH; ++k) {
-        const int u = tid + k * kNT;
-        if (u < NCHUNK) {
-            const int r = u / NCH, c4 = u - (u / NCH) * NCH;
-            ph[k] = *reinterpret_cast<const PFH_GLOBAL f32x4*>(
-                tin + (size_t)(TY0 + r) * kTilePitch + TX0 + 4 * c4);
+        for (int k = 0; k < KCH; ++k) {
+            const int u = tid + k * kNT;
+            if (u < NCHUNK) {
+                const int r = u / NCH, c4 = u - (u / NCH) * NCH;
+                ph[k] = *reinterpret_cast<const PFH_GLOBAL f32x4*>(
+                    tin + (size_t)(TY0 + r) * kTilePitch + TX0 + 4 * c4);
+            }
         }
     }
     // ... and this thread's column of the S window. S is addressed by a
@@ -502,14 +593,15 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         if (gy >= L) gy -= L;
         s_off0 = (uint32_t)gy * s_pitch + 4u * (uint32_t)gx;
     }
+    // Every lane issues all RB loads (control lanes and rows past the window
+    // read in-bounds S words that no row uses): with no branch around a load,
+    // each row of phase A waits only for its own word.
     auto load_qs = [&](uint32_t (&qS)[RB]) {
         uint32_t off = s_off0;
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
-            qS[k] = 0u;
-            if (active && row_ok(k))
-                qS[k] = *reinterpret_cast<const PFH_GLOBAL uint32_t*>(
-                    reinterpret_cast<const PFH_GLOBAL char*>(S_rd) + off);
+            qS[k] = *reinterpret_cast<const PFH_GLOBAL uint32_t*>(
+                reinterpret_cast<const PFH_GLOBAL char*>(S_rd) + off);
             off += s_pitch;
             if (off >= s_wrap) off -= s_wrap;
             asm volatile("" : "+v"(off));   // one live offset, not one per row
@@ -517,12 +609,19 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     };
     uint32_t qS[RB];
     load_qs(qS);
+    if (have) {
+        // This wave's global-to-LDS copies, issued during the previous cell,
+        // are older than the RB loads just issued: retire all but those.
+        static_assert(RB <= 63, "vmcnt is 6 bits");
+        asm volatile("s_waitcnt vmcnt(%0)" :: "n"(RB) : "memory");
+    } else {
 #pragma unroll
-    for (int k = 0; k < KCH; ++k) {
-        const int u = tid + k * kNT;
-        if (u < NCHUNK) {
-            const int r = u / NCH, c4 = u - (u / NCH) * NCH;
-            *reinterpret_cast<f32x4*>(phi_s + (r + 1) * PX + 4 + 4 * c4) = ph[k];
+        for (int k = 0; k < KCH; ++k) {
+            const int u = tid + k * kNT;
+            if (u < NCHUNK) {
+                const int r = u / NCH, c4 = u - (u / NCH) * NCH;
+                *reinterpret_cast<f32x4*>(phi_s + (r + 1) * PX + 4 + 4 * c4) = ph[k];
+            }
         }
     }
     // zero ring: the pads of every row (column -1 and column WX) and rows -1, WY
@@ -534,7 +633,12 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         }
     }
     lds_barrier();
-    if (control) fetch2(A, nf);
+    // The id and then the CellState of the next cell: wave 0 waits for them
+    // here, while the other waves of its SIMD start phase A.
+    if (control) {
+        fetch2(A, nf);
+        fetch3(A, nf);
+    }
 
     const float gam = c.gam, dwC = c.dwC, volC = c.volC;
     const float repC = A.rep_coeff;
@@ -545,8 +649,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     // ---- phase A: interaction integral and the held passive RHS ----
     float pas[RB];
     double aIx = 0.0, aIy = 0.0;
-    uint32_t negc = 0u;
     if (active) {
+        uint32_t negw = 0u;
         Roll q;
         load_row(col + y0 * PX, q.sW, q.sC, q.sE);          // row y0 - 1
         load_row(col + (y0 + 1) * PX, q.cW, q.cC, q.cE);    // row y0
@@ -556,7 +660,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             if (row_ok(k)) {
                 load_row(col + (y0 + k + 2) * PX, q.nW, q.nC, q.nE);
                 const float head = passive_head(q, gam, dwC, volC);
-                const float So = s_other_of(qS[k], q.cC, negc);
+                const float So = s_other_wave(qS[k], q.cC, negw);
                 if (wave_any(So != 0.0f)) {
                     pas[k] = head - repC * q.cC * So;
                     const float gxv = 0.5f * (q.cE - q.cW);
@@ -573,11 +677,14 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             // footprint at one row instead of the whole column.
             __builtin_amdgcn_sched_barrier(0);
         }
+        // two events per negative pixel (the reference evaluates S_other once
+        // per pass), added once per wave by its first active lane
+        if (negw && (int)__lane_id() == __builtin_ctzll(__builtin_amdgcn_ballot_w64(true)))
+            flag_add(A.flags, PFH_FLAG_S_NEGATIVE, 2u * negw);
     }
-    if (negc) flag_add(A.flags, PFH_FLAG_S_NEGATIVE, negc);
 
+    if (control) fetch4a(A, sm, nf, nxt);       // visible after the next barrier
     reduce_velocity(A, n, sm, c, aIx, aIy);
-    if (control) fetch3(A, nf);
 
     // ---- phase B: Euler update, shifted store, scatter, moments ----
     const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
@@ -597,9 +704,10 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         const bool ain = (unsigned)a < (unsigned)dwx;
         uint32_t olds[RB];
         int rlo = dwy, rhi = -1;
-        float sC = col[y0 * PX];
+        const float* cb = col + y0 * PX;          // row y0 - 1 of this column
+        float sC = cb[0];
         float cW, cC, cE;
-        load_row(col + (y0 + 1) * PX, cW, cC, cE);
+        load_row(cb + PX, cW, cC, cE);
         PFH_GLOBAL float* orow = tout + (ptrdiff_t)(dty0 + y0 - sy) * kTilePitch + dtx0 + a;
         // the aggregate's base in a VGPR pair: the per-row atomic then needs
         // neither the argument block nor a scalar wait
@@ -614,7 +722,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             olds[k] = 0u;
             if (row_ok(k)) {
                 float nW, nC, nE;
-                load_row(col + (y + 2) * PX, nW, nC, nE);
+                load_row(cb + (k + 2) * PX, nW, nC, nE);
                 const float gxv = 0.5f * (cE - cW);
                 const float gyv = 0.5f * (nC - sC);
                 const int b = y - sy;
@@ -640,6 +748,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 sC = cC;
                 cW = nW; cC = nC; cE = nE;
                 orow += kTilePitch;
+                asm volatile("" : "+v"(orow));   // a pointer increment, not a multiply
                 s_off += s_pitch;
                 if (s_off >= s_wrap) s_off -= s_wrap;
                 asm volatile("" : "+v"(s_off));
@@ -743,9 +852,21 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             aPer += (double)sqrtf(pgx * pgx + pgy * pgy);
         }
     }
-    if (control) fetch4(A, sm, nf, nxt, step);
-    reduce_commit<CLS>(A, n, sm, c, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
-                       pmaxb);
+    // The next cell's scalars, on a thread of wave 1: the first waves of each
+    // SIMD finish phase B early, wave 0 (control lanes, commit) the latest.
+    if (tid == kScalarsThread) fetch4b(A, sm, nxt, step);
+    if constexpr (CLS == 0) {
+        reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
+                           blox, bhix, bloy, bhiy, pmaxb, [&]() {
+            const int ncls = __builtin_amdgcn_readfirstlane(sm.slot[nxt].cls);
+            const int nn = __builtin_amdgcn_readfirstlane(sm.slot[nxt].n);
+            pre = (ncls == 0 && nn >= 0) ? 1 : 0;
+            if (pre) prefetch_window0(A, sm, nn);
+        });
+    } else {
+        reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
+                           blox, bhix, bloy, bhiy, pmaxb);
+    }
 }
 
 // -------------------------------------------------------------------------
@@ -867,8 +988,8 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
     if (nonfin) flag_or(A.flags, PFH_FLAG_NONFINITE);
     if (clampf) flag_or(A.flags, PFH_FLAG_Q_CLAMP);
     if (ovf) flag_or(A.flags, PFH_FLAG_S_OVERFLOW);
-    reduce_commit<CLS>(A, n, sm, c, aV, aCx, aCy, aPer, blox, bhix, bloy, bhiy,
-                       pmaxb);
+    reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
+                       blox, bhix, bloy, bhiy, pmaxb);
 }
 
 // Classes other than the round class 0 run out of line, with their own
@@ -877,8 +998,10 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
 template <int CLS>
 __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
                                        unsigned long long step, Smem* sm, int cur) {
-    if constexpr (CLS <= kClassStagedMax)
-        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
+    if constexpr (CLS <= kClassStagedMax) {
+        int none = 0;
+        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none);
+    }
     else
         cell_global<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
 }
@@ -919,12 +1042,13 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     if (tid < kControlLanes) fetch_all(A, sm, 0, step);
     lds_barrier();
     int cur = 0;
+    int pre = 0;                       // the current class-0 window is already in LDS
     for (;;) {
         const int cls = __builtin_amdgcn_readfirstlane(sm.slot[cur].cls);
         const int n = __builtin_amdgcn_readfirstlane(sm.slot[cur].n);
         if (cls < 0) break;
         switch (cls) {
-            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1); break;
+            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre); break;
             case 1: cell_cold<1>(&sm.args, n, step, &sm, cur); break;
             case 2: cell_cold<2>(&sm.args, n, step, &sm, cur); break;
             case 3: cell_cold<3>(&sm.args, n, step, &sm, cur); break;
@@ -934,6 +1058,12 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
                 if (tid < kControlLanes) fetch_all(A, sm, cur ^ 1, step);
                 lds_barrier();
                 break;
+        }
+        if (cls != 0) {
+            pre = 0;
+            // Retire the out-of-line paths' register reloads here, so that the
+            // loop head does not wait on them (with everything else in flight).
+            __builtin_amdgcn_s_waitcnt(0);
         }
         cur ^= 1;
     }

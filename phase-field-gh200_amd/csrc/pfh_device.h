@@ -79,6 +79,7 @@ template <int CLS> struct Staged {
     static constexpr int TY0 = kShapes[CLS].ty0;
     static constexpr int NB = kPixThreads / WX;
     static constexpr int RB = (WY + NB - 1) / NB;
+    static constexpr int RFULL = WY - (NB - 1) * RB;    // rows every band holds
     static constexpr int NCH = WX / 4;                  // 16 B chunks per row
     static constexpr int NCHUNK = WY * NCH;
     static constexpr int KCH = (NCHUNK + kNT - 1) / kNT;
