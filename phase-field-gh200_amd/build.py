@@ -31,6 +31,13 @@ HIP_FLAGS = [
     "-I", str(REPO / "include"), "-I", str(CSRC),
 ]
 
+# Per-source additions. The 2D step kernel runs at the 128-VGPR limit of a
+# 1024-thread workgroup; sinking definitions towards their uses removes most
+# of its scratch spills (each spill reload waits for every load in flight).
+FILE_FLAGS = {
+    "pf2d_kernels.hip": ["-mllvm", "-sink-insts-to-avoid-spills"],
+}
+
 
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,7 +64,8 @@ def build(force: bool = True) -> Path:
         obj = LIB / (src.stem + (".hip.o" if src.suffix == ".hip" else ".o"))
         if force or _stale(obj, [src] + headers):
             lang = [] if src.suffix == ".hip" else ["-x", "hip"]
-            _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(src), "-o", str(obj)])
+            _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src.name, []), *lang, "-c", str(src),
+                  "-o", str(obj)])
         objs.append(obj)
     so = LIB / "libpfh.so"
     tmp = LIB / "libpfh.so.tmp"
