@@ -513,26 +513,27 @@ __device__ __forceinline__ float s_other_wave(uint32_t qS, float c, uint32_t& ne
     return neg ? 0.0f : (float)(qS - qc) * kQInv;
 }
 
-// Class-0 window prefetch: once a class-0 cell's pixel phases are done with
-// the LDS window, and the next cell in the queue is also of class 0, the
-// next window's rows are copied from HBM straight into LDS (global-to-LDS
-// loads, no registers) while the current cell's moments are committed. Wave
-// w >= 1 copies rows w - 1, w + 14, ...; one instruction moves one 576 B row
-// with 36 lanes of 16 B into the row's place in the padded layout. Wave 0
-// (thread 0 commits the cell, the control lanes fetch) copies nothing, so
-// that none of its own memory waits has to wait for the copies.
-__device__ __forceinline__ void prefetch_window0(const pfh2d_step_args& A, Smem& sm,
-                                                 int nn) {
+// Class-0 window prefetch: rows [r0, r1) of cell nn's class-0 window,
+// copied from HBM straight into LDS (global-to-LDS loads, no registers), row
+// r to dst0 + (r - r0) * PX. Waves W0..W0+NW-1 take rows r0 + (w - W0) + NW j;
+// one instruction moves one 576 B row with 36 lanes of 16 B. Wave 0 (thread
+// 0 commits the cell, the control lanes fetch) is never among them, so that
+// none of its own memory waits has to wait for the copies. A wave's copies
+// are complete when its vmcnt has passed them; LDS release fences (and so
+// every barrier here) wait for them too.
+template <int W0 = 1, int NW = kWaves - 1>
+__device__ __forceinline__ void copy_rows0(const pfh2d_step_args& A, int nn, float* dst0,
+                                           int r0, int r1) {
     using G = Staged<0>;
     const int lane = (int)threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const PFH_GLOBAL float* src = gptr(A.phi_in) + (size_t)nn * kTileArea
                                 + (size_t)G::TY0 * kTilePitch + G::TX0 + 4 * lane;
-    if (wave > 0 && lane < G::NCH) {
-        for (int r = wave - 1; r < G::WY; r += kWaves - 1)
+    if (wave >= W0 && wave < W0 + NW && lane < G::NCH) {
+        for (int r = r0 + wave - W0; r < r1; r += NW)
             __builtin_amdgcn_global_load_lds(
                 (const PFH_GLOBAL void*)(src + (size_t)r * kTilePitch),
-                (__attribute__((address_space(3))) void*)(s_window + (r + 1) * G::PX + 4),
+                (__attribute__((address_space(3))) void*)(dst0 + (r - r0) * G::PX),
                 16, 0, 0);
     }
 }
@@ -540,7 +541,7 @@ __device__ __forceinline__ void prefetch_window0(const pfh2d_step_args& A, Smem&
 template <int CLS>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt,
-                            int& pre) {
+                            int& pre, int& pslot) {
     using G = Staged<CLS>;
     constexpr int WX = G::WX, WY = G::WY, TX0 = G::TX0, TY0 = G::TY0;
     constexpr int PX = G::PX;
@@ -561,7 +562,26 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     const int gx0 = c.gx0;
     const int gy0 = c.gy0;
     float* phi_s = s_window;
-    const float* col = phi_s + 4 + x;           // row r of column x: col[r*PX]
+    // The window in LDS: class 0 in its two pieces (piece 1 in slot pslot),
+    // the other classes whole. rowp(r) is the start of row r's data; cbw is
+    // this thread's column at row y0 - 1 (a band's rows are all in one piece).
+    float* base1 = phi_s;
+    float* base2 = phi_s;
+    if constexpr (CLS == 0) {
+        base1 = phi_s + pslot * Split0::P1_FLOATS;
+        base2 = phi_s + 2 * Split0::P1_FLOATS;
+    }
+    auto rowp = [&](int r) -> float* {
+        if constexpr (CLS == 0)
+            return r <= Split0::P2_R0 ? base1 + (r + 1) * PX + 4
+                                      : base2 + (r - Split0::P2_R0) * PX + 4;
+        else
+            return phi_s + (r + 1) * PX + 4;
+    };
+    float* cbw = base1 + y0 * PX + 4 + x;
+    if constexpr (CLS == 0) {
+        if (band >= Split0::KB) cbw = base2 + (y0 - 1 - Split0::P2_R0) * PX + 4 + x;
+    }
     const int gx = (gx0 + x < L) ? gx0 + x : gx0 + x - L;
     // k is a compile-time row index: rows below RFULL exist in every band, so
     // only the last band's tail rows are tested at run time.
@@ -620,16 +640,39 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             const int u = tid + k * kNT;
             if (u < NCHUNK) {
                 const int r = u / NCH, c4 = u - (u / NCH) * NCH;
-                *reinterpret_cast<f32x4*>(phi_s + (r + 1) * PX + 4 + 4 * c4) = ph[k];
+                if constexpr (CLS == 0) {             // rows P2_R0, R1 go to both pieces
+                    if (r <= Split0::R1)
+                        *reinterpret_cast<f32x4*>(base1 + (r + 1) * PX + 4 + 4 * c4) = ph[k];
+                    if (r >= Split0::P2_R0)
+                        *reinterpret_cast<f32x4*>(base2 + (r - Split0::P2_R0) * PX + 4 + 4 * c4) = ph[k];
+                } else {
+                    *reinterpret_cast<f32x4*>(phi_s + (r + 1) * PX + 4 + 4 * c4) = ph[k];
+                }
             }
         }
     }
     // zero ring: the pads of every row (column -1 and column WX) and rows -1, WY
-    for (int i = tid; i < (WY + 3) * 4 + 2 * WX; i += kNT) {
-        if (i < (WY + 3) * 4) phi_s[(i >> 2) * PX + (i & 3)] = 0.0f;
-        else {
-            const int j = i - (WY + 3) * 4;
-            phi_s[(j < WX ? 0 : (WY + 1) * PX - WX) + 4 + j] = 0.0f;
+    if constexpr (CLS == 0) {
+        constexpr int n1 = (Split0::P1_ROWS + 1) * 4, n2 = (Split0::P2_ROWS + 1) * 4;
+        for (int i = tid; i < n1 + n2 + 2 * WX; i += kNT) {
+            if (i < n1) {
+                base1[(i >> 2) * PX + (i & 3)] = 0.0f;
+            } else if (i < n1 + n2) {
+                const int j = i - n1;
+                base2[(j >> 2) * PX + (j & 3)] = 0.0f;
+            } else {
+                const int j = i - n1 - n2;
+                if (j < WX) base1[4 + j] = 0.0f;
+                else base2[(WY - Split0::P2_R0) * PX + 4 + (j - WX)] = 0.0f;
+            }
+        }
+    } else {
+        for (int i = tid; i < (WY + 3) * 4 + 2 * WX; i += kNT) {
+            if (i < (WY + 3) * 4) phi_s[(i >> 2) * PX + (i & 3)] = 0.0f;
+            else {
+                const int j = i - (WY + 3) * 4;
+                phi_s[(j < WX ? 0 : (WY + 1) * PX - WX) + 4 + j] = 0.0f;
+            }
         }
     }
     lds_barrier();
@@ -652,13 +695,13 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     if (active) {
         uint32_t negw = 0u;
         Roll q;
-        load_row(col + y0 * PX, q.sW, q.sC, q.sE);          // row y0 - 1
-        load_row(col + (y0 + 1) * PX, q.cW, q.cC, q.cE);    // row y0
+        load_row(cbw, q.sW, q.sC, q.sE);
+        load_row(cbw + PX, q.cW, q.cC, q.cE);
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             pas[k] = 0.0f;
             if (row_ok(k)) {
-                load_row(col + (y0 + k + 2) * PX, q.nW, q.nC, q.nE);
+                load_row(cbw + (k + 2) * PX, q.nW, q.nC, q.nE);
                 const float head = passive_head(q, gam, dwC, volC);
                 const float So = s_other_wave(qS[k], q.cC, negw);
                 if (wave_any(So != 0.0f)) {
@@ -685,6 +728,15 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
 
     if (control) fetch4a(A, sm, nf, nxt);       // visible after the next barrier
     reduce_velocity(A, n, sm, c, aIx, aIy);
+    // A class-0 cell next in the queue gets its window copied in ahead:
+    // piece 1 into the other slot (its last reader was the previous cell) as
+    // soon as the first waves finish phase B, piece 2 once every wave has.
+    int next_nn = -1;
+    if constexpr (CLS == 0) {
+        const int ncls = __builtin_amdgcn_readfirstlane(sm.slot[nxt].cls);
+        const int nn = __builtin_amdgcn_readfirstlane(sm.slot[nxt].n);
+        if (ncls == 0 && nn >= 0) next_nn = nn;
+    }
 
     // ---- phase B: Euler update, shifted store, scatter, moments ----
     const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
@@ -704,7 +756,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         const bool ain = (unsigned)a < (unsigned)dwx;
         uint32_t olds[RB];
         int rlo = dwy, rhi = -1;
-        const float* cb = col + y0 * PX;          // row y0 - 1 of this column
+        const float* cb = cbw;
         float sC = cb[0];
         float cW, cC, cE;
         load_row(cb + PX, cW, cC, cE);
@@ -810,13 +862,13 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             load_qs(qS2);
             uint32_t negc2 = 0u;
             Roll q;
-            load_row(col + y0 * PX, q.sW, q.sC, q.sE);
-            load_row(col + (y0 + 1) * PX, q.cW, q.cC, q.cE);
+            load_row(cbw, q.sW, q.sC, q.sE);
+            load_row(cbw + PX, q.cW, q.cC, q.cE);
 #pragma unroll
             for (int k = 0; k < RB; ++k) {
                 pns[k] = 0.0f;
                 if (row_ok(k)) {
-                    load_row(col + (y0 + k + 2) * PX, q.nW, q.nC, q.nE);
+                    load_row(cbw + (k + 2) * PX, q.nW, q.nC, q.nE);
                     const float So = s_other_of(qS2[k], q.cC, negc2);
                     const float pk = passive_head(q, gam, dwC, volC) - repC * q.cC * So;
                     pns[k] = update(q.cC, 0.5f * (q.cE - q.cW), 0.5f * (q.nC - q.sC), pk);
@@ -830,7 +882,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         if (active) {
 #pragma unroll
             for (int k = 0; k < RB; ++k)
-                if (row_ok(k)) phi_s[(y0 + k + 1) * PX + 4 + x] = pns[k];
+                if (row_ok(k)) cbw[(k + 1) * PX] = pns[k];
         }
         lds_barrier();
         for (int i = tid; i < dwx * dwy; i += kNT) {
@@ -838,15 +890,15 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             const int ii = aa + sx, jj = b + sy;
             const bool iin = (unsigned)ii < (unsigned)WX;
             const bool jin = (unsigned)jj < (unsigned)WY;
-            const float* pr = phi_s + (jj + 1) * PX + 4 + ii;
             float e = 0.0f, w = 0.0f, nn = 0.0f, ss = 0.0f;
             if (jin) {
+                const float* pr = rowp(jj) + ii;
                 if ((unsigned)(ii + 1) < (unsigned)WX) e = pr[1];
                 if ((unsigned)(ii - 1) < (unsigned)WX) w = pr[-1];
             }
             if (iin) {
-                if ((unsigned)(jj + 1) < (unsigned)WY) nn = pr[PX];
-                if ((unsigned)(jj - 1) < (unsigned)WY) ss = pr[-PX];
+                if ((unsigned)(jj + 1) < (unsigned)WY) nn = rowp(jj + 1)[ii];
+                if ((unsigned)(jj - 1) < (unsigned)WY) ss = rowp(jj - 1)[ii];
             }
             const float pgx = 0.5f * (e - w), pgy = 0.5f * (nn - ss);
             aPer += (double)sqrtf(pgx * pgx + pgy * pgy);
@@ -856,13 +908,19 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     // SIMD finish phase B early, wave 0 (control lanes, commit) the latest.
     if (tid == kScalarsThread) fetch4b(A, sm, nxt, step);
     if constexpr (CLS == 0) {
+        // piece 1 of the next window, by waves 1-7: the first waves of each
+        // SIMD, which finish phase B first
+        if (next_nn >= 0)
+            copy_rows0<1, 7>(A, next_nn, phi_s + (pslot ^ 1) * Split0::P1_FLOATS + PX + 4, 0,
+                             Split0::R1 + 1);
+    }
+    if constexpr (CLS == 0) {
         reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
                            blox, bhix, bloy, bhiy, pmaxb, [&]() {
-            const int ncls = __builtin_amdgcn_readfirstlane(sm.slot[nxt].cls);
-            const int nn = __builtin_amdgcn_readfirstlane(sm.slot[nxt].n);
-            pre = (ncls == 0 && nn >= 0) ? 1 : 0;
-            if (pre) prefetch_window0(A, sm, nn);
+            if (next_nn >= 0) copy_rows0(A, next_nn, base2 + 4, Split0::P2_R0, WY);
         });
+        pre = next_nn >= 0 ? 1 : 0;
+        if (pre) pslot ^= 1;
     } else {
         reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
                            blox, bhix, bloy, bhiy, pmaxb);
@@ -999,8 +1057,8 @@ template <int CLS>
 __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
                                        unsigned long long step, Smem* sm, int cur) {
     if constexpr (CLS <= kClassStagedMax) {
-        int none = 0;
-        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none);
+        int none = 0, slot0 = 0;
+        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none, slot0);
     }
     else
         cell_global<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
@@ -1043,12 +1101,13 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     lds_barrier();
     int cur = 0;
     int pre = 0;                       // the current class-0 window is already in LDS
+    int pslot = 0;                     // slot of the class-0 window's piece 1
     for (;;) {
         const int cls = __builtin_amdgcn_readfirstlane(sm.slot[cur].cls);
         const int n = __builtin_amdgcn_readfirstlane(sm.slot[cur].n);
         if (cls < 0) break;
         switch (cls) {
-            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre); break;
+            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre, pslot); break;
             case 1: cell_cold<1>(&sm.args, n, step, &sm, cur); break;
             case 2: cell_cold<2>(&sm.args, n, step, &sm, cur); break;
             case 3: cell_cold<3>(&sm.args, n, step, &sm, cur); break;

@@ -89,8 +89,29 @@ template <int CLS> struct Staged {
     static_assert(NB * RB >= WY, "bands must cover the window");
 };
 constexpr int max2(int a, int b) { return a > b ? a : b; }
+
+// Class 0 keeps its LDS window in two pieces, so that the next cell's window
+// can be copied in while the current one is still being read: piece 1 holds
+// rows -1..R1 (read by bands 0..KB-1) in one of two slots, piece 2 rows
+// P2_R0..WY (read by bands KB..NB-1). Rows P2_R0 and R1 are in both pieces.
+// Each piece has the padded row layout of the whole window, with four floats
+// after its last row (that row's east pad).
+struct Split0 {
+    using G = Staged<0>;
+    static constexpr int KB = 5;
+    static constexpr int R1 = KB * G::RB;              // first row of band KB
+    static constexpr int P2_R0 = R1 - 1;
+    static constexpr int P1_ROWS = R1 + 2;             // rows -1..R1
+    static constexpr int P2_ROWS = G::WY + 1 - P2_R0;  // rows P2_R0..WY
+    static constexpr int P1_FLOATS = P1_ROWS * G::PX + 4;
+    static constexpr int P2_FLOATS = P2_ROWS * G::PX + 4;
+    static constexpr int FLOATS = 2 * P1_FLOATS + P2_FLOATS;
+    static_assert(KB < G::NB && (G::NB - 1) * G::RB < G::WY, "two non-empty pieces");
+    static_assert(P1_FLOATS % 4 == 0 && P2_FLOATS % 4 == 0, "16 B piece alignment");
+};
+
 constexpr int kPhiLdsFloats =
-    max2(max2(Staged<0>::LDS_FLOATS, Staged<1>::LDS_FLOATS),
+    max2(max2(Split0::FLOATS, Staged<1>::LDS_FLOATS),
          max2(Staged<2>::LDS_FLOATS, Staged<3>::LDS_FLOATS));
 
 // ---- Q5.27 ----
