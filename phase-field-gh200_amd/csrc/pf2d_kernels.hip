@@ -908,11 +908,11 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     // SIMD finish phase B early, wave 0 (control lanes, commit) the latest.
     if (tid == kScalarsThread) fetch4b(A, sm, nxt, step);
     if constexpr (CLS == 0) {
-        // piece 1 of the next window, by waves 1-7: the first waves of each
-        // SIMD, which finish phase B first
+        // piece 1 of the next window: each wave issues its rows as it
+        // finishes phase B
         if (next_nn >= 0)
-            copy_rows0<1, 7>(A, next_nn, phi_s + (pslot ^ 1) * Split0::P1_FLOATS + PX + 4, 0,
-                             Split0::R1 + 1);
+            copy_rows0(A, next_nn, phi_s + (pslot ^ 1) * Split0::P1_FLOATS + PX + 4, 0,
+                       Split0::R1 + 1);
     }
     if constexpr (CLS == 0) {
         reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
