@@ -62,7 +62,16 @@ using pfh::uniform53;
 constexpr uint32_t kFatal = PFH3D_FATAL_MASK;
 constexpr float kEps = 1.0e-5f;
 constexpr int kMargin = 5;        // LDS zero margin of the update ring (shift 4 + 1)
-constexpr int kMarginX = 8;       // keeps the interior 16 B aligned
+constexpr int kMarginG = 2;       // the same in x, in 16 B groups (x >= -8)
+
+// Update ring rows: four sub-rows (one per x mod 4) of B/4 + 2 kMarginG words
+// each, indexed by the group x >> 2. The pitch is the smallest one that holds
+// them and is congruent to B/4 modulo 64, so that the lanes of a wave, B/4
+// per row, address consecutive banks across rows too.
+__host__ __device__ inline int update_row_pitch(int B) {
+    const int span = 4 * (B / 4 + 2 * kMarginG);
+    return span + ((B / 4 - span) % 64 + 64) % 64;
+}
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -621,14 +630,32 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     kc.dt = a.dt;
     kc.ic = a.interaction_coeff;
 
-    const int PXU = B + 2 * kMarginX - 4;            // B + 12
+    const int GPU = B / 4 + 2 * kMarginG;            // words per sub-row
+    const int PXU = update_row_pitch(B);
     const int PLU = (B + 2 * kMargin + 1) * PXU;      // B + 11 rows
     for (int i = tid; i < 2 * PLU; i += NT) lds[i] = 0.0f;
 
-    // staging: lane tid owns source group (4 (tid % gpr), tid / gpr)
-    const int stage = (gy_ + kMargin) * PXU + kMarginX + gx0;
-    // reading: destination group (gx0.., gy_) reads source (gx0 + sx.., gy_ + sy)
-    const int rd = (gy_ + sy + kMargin) * PXU + kMarginX + gx0 + sx;
+    // staging: lane tid owns source group g = tid % gpr of row tid / gpr;
+    // element e of the group goes to sub-row e
+    const int g = gx0 >> 2;
+    const int stage = (gy_ + kMargin) * PXU + kMarginG + g;
+    // reading: destination group (gx0.., gy_) reads source x = gx0 + sx + c - 1,
+    // c = 0..5, of rows gy_ + sy - 1..gy_ + sy + 1: sub-row (sx + c - 1) & 3,
+    // group g + ((sx + c - 1) >> 2), both offsets uniform
+    const int rd = (gy_ + sy + kMargin) * PXU + kMarginG + g;
+    int roff[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const int t = sx + c - 1;
+        roff[c] = (t & 3) * GPU + (t >> 2);
+    }
+    auto stage_plane = [&](float* plane, const f32x4& v) {
+        float* st = plane + stage;
+        st[0] = v[0];
+        st[GPU] = v[1];
+        st[2 * GPU] = v[2];
+        st[3 * GPU] = v[3];
+    };
     auto load_plane = [&](int p) -> f32x4 {
         f32x4 v = {0.f, 0.f, 0.f, 0.f};
         if (act && (unsigned)p < (unsigned)B)
@@ -641,7 +668,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     f32x4 R = load_plane(pbeg);
     f32x4 Rn = load_plane(pbeg + 1);
     __syncthreads();
-    if (act) *reinterpret_cast<f32x4*>(lds + stage) = R;
+    if (act) stage_plane(lds, R);
     __syncthreads();
 
     float T[4] = {0.f, 0.f, 0.f, 0.f}, Q1p[4] = {0.f, 0.f, 0.f, 0.f};
@@ -693,9 +720,9 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
             float r0[6], r1[6], r2[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-                r0[c] = pl[c - 1 - PXU];
-                r1[c] = pl[c - 1];
-                r2[c] = pl[c - 1 + PXU];
+                r0[c] = pl[roff[c] - PXU];
+                r1[c] = pl[roff[c]];
+                r2[c] = pl[roff[c] + PXU];
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -757,7 +784,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
             gxm[j] = GX[j];
             gym[j] = GY[j];
         }
-        if (i + 1 < pcount && act) *reinterpret_cast<f32x4*>(lds + (cur ^ 1) * PLU + stage) = Rn;
+        if (i + 1 < pcount && act) stage_plane(lds + (cur ^ 1) * PLU, Rn);
         pfh::lds_barrier();   // the plane buffers are the only shared data here
         Rn = Rnn;
 #pragma unroll
@@ -944,7 +971,8 @@ int block_threads(int B) {
 }
 size_t measure_lds(int B) { return 2u * (size_t)(B + 2) * (size_t)(B + 4) * sizeof(float); }
 size_t update_lds(int B) {
-    return 2u * (size_t)(B + 2 * pfh3::kMargin + 1) * (size_t)(B + 12) * sizeof(float);
+    return 2u * (size_t)(B + 2 * pfh3::kMargin + 1) * (size_t)pfh3::update_row_pitch(B) *
+           sizeof(float);
 }
 
 int check_geometry(const pfh3d_step_args* a) {
