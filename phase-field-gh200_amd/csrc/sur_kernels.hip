@@ -30,6 +30,7 @@
 
 namespace sur {
 
+typedef float fx4 __attribute__((ext_vector_type(4)));
 constexpr int kCrop = 96;
 constexpr int kCropPix = kCrop * kCrop;
 constexpr int kC0 = 8;     // input channels (4 used)
@@ -933,23 +934,39 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
     if (go && *go == 0) return;
     __shared__ double r1[4], r2[4];
     const int n = (int)blockIdx.x;
-    const pfh_cell_state c = cells[n];
+    const int gx0 = cells[n].gx0, gy0 = cells[n].gy0;
     const int ox = origin[2 * n], oy = origin[2 * n + 1];
     const float* tile = phi_chk + (size_t)n * pfh::kTileArea;
-    const int k = (int)cls[n];
+    const pfh::Shape sh = pfh::shape_of((int)cls[n]);
+    // crop column 0 and row 0 relative to the cell's active window
+    int a0 = ox - gx0;
+    if (a0 < 0) a0 += L;
+    int b0 = oy - gy0;
+    if (b0 < 0) b0 += L;
     double e = 0.0, s = 0.0;
-    // unrolled so that several pixels' loads are in flight (sums in order)
-#pragma unroll 6
-    for (int p = (int)threadIdx.x; p < kCropPix; p += 256) {
-        const int i = p % kCrop, j = p / kCrop;
-        int wx = ox + i;
-        if (wx >= L) wx -= L;
-        int wy = oy + j;
-        if (wy >= L) wy -= L;
-        const double b = (double)cell_phi(tile, c, k, L, wx, wy);
-        const double d = (double)pred[(size_t)n * kCropPix + p] - b;
-        e += d * d;
-        s += b * b;
+    // item t = 4 pixels of one crop row (one 16 B load of the prediction);
+    // 96 x 24 items, 9 per thread, all in flight together
+    constexpr int kQuads = kCrop / 4, kItems = kCrop * kQuads;
+    static_assert(kItems % 256 == 0, "items per thread");
+#pragma unroll
+    for (int r = 0; r < kItems / 256; ++r) {
+        const int t = (int)threadIdx.x + 256 * r;
+        const int j = t / kQuads, i0 = 4 * (t - j * kQuads);
+        const fx4 pv = *reinterpret_cast<const fx4*>(pred + (size_t)n * kCropPix + j * kCrop + i0);
+        int b = b0 + j;
+        if (b >= L) b -= L;
+        const bool brow = b < sh.wy;
+        const float* trow = tile + (size_t)(sh.ty0 + (brow ? b : 0)) * pfh::kTilePitch + sh.tx0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int a = a0 + i0 + u;
+            if (a >= L) a -= L;
+            const float bv = (brow && a < sh.wx) ? trow[a] : 0.0f;
+            const double bd = (double)bv;
+            const double d = (double)pv[u] - bd;
+            e += d * d;
+            s += bd * bd;
+        }
     }
     e = pfh::wave_sum(e);
     s = pfh::wave_sum(s);
