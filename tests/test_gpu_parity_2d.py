@@ -143,6 +143,42 @@ def test_all_classes_forced_paths_bit_exact(sim_mod, oracle, step0):
     assert sim.flag_counts()[abi.FLAG_NAMES.index("class_exhausted")] > 0
 
 
+@pytest.mark.parametrize("grid,step0", [(1, 0), (2, 99), (3, 5)])
+def test_few_workgroups_mixed_classes_bit_exact(sim_mod, oracle, grid, step0):
+    """The same forced state on 1-3 workgroups, so that each one takes several
+    cells of mixed classes in queue order: class-0 windows copied in ahead of
+    the next class-0 cell (and the slot of the double-buffered piece), the
+    switch to and from the out-of-line classes, and at step 99 the perimeter
+    pass of a full-moment step."""
+    wl, phi, S, cells, cls = _forced_state(oracle)
+    say(f"forced all-classes state L=701 N=12 on {grid} workgroup(s), start step {step0}")
+    sim = sim_mod.Sim2D(wl)
+    sim.grid = grid
+    orc = oracle.OracleSim2D(wl, threads=8)
+    sim.load_state(phi, S, cells, cls, step0)
+    orc.load_state(phi, S, cells, cls, step0)
+    for k in range(3):
+        sim.step(1)
+        orc.step(1)
+        sim.sync()
+        assert_state_equal(sim, orc, f"grid {grid}, step {step0 + k + 1}")
+
+
+def test_c1_two_workgroups_bit_exact(sim_mod, oracle):
+    """C1's 16 class-0 cells on two workgroups: eight cells in a row per
+    workgroup, each window copied in during the previous cell."""
+    wl = params.workload_c1()
+    say("C1 on 2 workgroups, 20 steps vs oracle")
+    sim = sim_mod.Sim2D(wl)
+    sim.grid = 2
+    orc = oracle.OracleSim2D(wl, threads=8)
+    for k in range(2):
+        sim.step(10)
+        orc.step(10)
+        sim.sync()
+        assert_state_equal(sim, orc, f"C1 two workgroups, step {sim.steps_done}")
+
+
 def test_fatal_flags_match_oracle(sim_mod, oracle):
     wl, phi, S, cells, cls = _forced_state(oracle)
     cells = cells.copy()
