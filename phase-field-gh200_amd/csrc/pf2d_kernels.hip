@@ -504,12 +504,12 @@ __device__ __forceinline__ float s_other_of(uint32_t qS, float c, uint32_t& negc
     return neg ? 0.0f : (float)(qS - qc) * kQInv;
 }
 
-// The same, counting the wave's negative differences in a scalar register
-// (the count of the active lanes whose S is below their own q).
-__device__ __forceinline__ float s_other_wave(uint32_t qS, float c, uint32_t& negw) {
+// The same, counting this lane's negative differences once each (the
+// subtraction's borrow is added in with carry).
+__device__ __forceinline__ float s_other_lane(uint32_t qS, float c, uint32_t& neg1) {
     const uint32_t qc = q_of(c);
     const bool neg = qS < qc;
-    negw += (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(neg));
+    neg1 += (uint32_t)neg;
     return neg ? 0.0f : (float)(qS - qc) * kQInv;
 }
 
@@ -693,7 +693,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     float pas[RB];
     double aIx = 0.0, aIy = 0.0;
     if (active) {
-        uint32_t negw = 0u;
+        uint32_t neg1 = 0u;
         Roll q;
         load_row(cbw, q.sW, q.sC, q.sE);
         load_row(cbw + PX, q.cW, q.cC, q.cE);
@@ -703,7 +703,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             if (row_ok(k)) {
                 load_row(cbw + (k + 2) * PX, q.nW, q.nC, q.nE);
                 const float head = passive_head(q, gam, dwC, volC);
-                const float So = s_other_wave(qS[k], q.cC, negw);
+                const float So = s_other_lane(qS[k], q.cC, neg1);
                 if (wave_any(So != 0.0f)) {
                     pas[k] = head - repC * q.cC * So;
                     const float gxv = 0.5f * (q.cE - q.cW);
@@ -721,9 +721,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             __builtin_amdgcn_sched_barrier(0);
         }
         // two events per negative pixel (the reference evaluates S_other once
-        // per pass), added once per wave by its first active lane
-        if (negw && (int)__lane_id() == __builtin_ctzll(__builtin_amdgcn_ballot_w64(true)))
-            flag_add(A.flags, PFH_FLAG_S_NEGATIVE, 2u * negw);
+        // per pass)
+        if (neg1) flag_add(A.flags, PFH_FLAG_S_NEGATIVE, 2u * neg1);
     }
 
     if (control) fetch4a(A, sm, nf, nxt);       // visible after the next barrier
