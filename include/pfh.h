@@ -198,6 +198,12 @@ int pfh2d_launch_step(const pfh2d_step_args* args, int grid, void* stream,
  * device-side decision (the check step runs only while the block is live). */
 int pfh2d_launch_step_if(const pfh2d_step_args* args, const int* run_if, int grid,
                          void* stream);
+/* pfh2d_launch_step_if for a surrogate check step (no reference
+ * counterpart): the same new field in phi_out, and each cell's new window
+ * position and class (gx0, gy0, cls, cls_written, cell_cls); no other
+ * CellState field, flag or aggregate is written. S_sc must be NULL. */
+int pfh2d_launch_check_step_if(const pfh2d_step_args* args, const int* run_if, int grid,
+                               void* stream);
 
 /* Replaces query_kernel_stats (include/kernels.cuh:243-244) for k_step. */
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out);

@@ -373,6 +373,24 @@ __device__ void reduce_commit(const pfh2d_step_args& A, int n, Smem& sm,
     lds_barrier();
 }
 
+// The check step's commit: the cell's new window position and class (all its
+// consumer reads), between the same two barriers as reduce_commit.
+template <class Hook>
+__device__ void commit_position(const pfh2d_step_args& A, int n, Smem& sm, const Ctl& c,
+                                Hook after_partials) {
+    lds_barrier();
+    after_partials();
+    if (threadIdx.x == 0) {
+        pfh_cell_state* cs = &A.cell[n];
+        cs->gx0 = c.gx0n;
+        cs->gy0 = c.gy0n;
+        cs->cls = (uint8_t)c.dcls;
+        cs->cls_written[A.parity_out] = (uint8_t)c.dcls;
+        A.cell_cls[n] = (uint8_t)c.dcls;
+    }
+    lds_barrier();
+}
+
 // -------------------------------------------------------------------------
 // Next-cell fetch by the control lanes, in four stages so that each stage
 // only waits on a load issued one phase earlier:
@@ -538,7 +556,10 @@ __device__ __forceinline__ void copy_rows0(const pfh2d_step_args& A, int nn, flo
     }
 }
 
-template <int CLS>
+// kCheck: a check step (the surrogate's acceptance field), which needs the
+// new field and the cell's window position only: no scatter, moments,
+// support, flags or perimeter, and a commit of the position alone.
+template <int CLS, bool kCheck = false>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt,
                             int& pre, int& pslot) {
@@ -780,6 +801,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 const bool valid = ain && (unsigned)b < (unsigned)dwy;
                 const float pn = update(cC, gxv, gyv, pas[k]);
                 if (valid) *orow = pn;
+                if constexpr (!kCheck) {
                 // A lane whose destination is outside the window contributes
                 // an exact zero to every sum below.
                 const float pv = valid ? pn : 0.0f;
@@ -796,6 +818,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 aV += d;
                 aCy += d * (double)b;
                 if (pv > kSupportEps) { rlo = min(rlo, b); rhi = max(rhi, b); }
+                }
                 sC = cC;
                 cW = nW; cC = nC; cE = nE;
                 orow += kTilePitch;
@@ -812,7 +835,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         // at most 2^29, so a wrap needs old >= 2^32 - 2^29; only such
         // candidates recompute q from this thread's own stored value.
 #pragma unroll
-        for (int k = 0; k < RB; ++k) {
+        for (int k = 0; k < (kCheck ? 0 : RB); ++k) {
             if (olds[k] >= 0xE0000000u) {
                 const int b = y0 + k - sy;
                 const uint32_t qv = q_of(tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a]);
@@ -854,7 +877,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     // new field in the source layout with source-window zeros (the staged
     // semantics, src/kernels.cu:512-538). phi_next is recomputed here, bit for
     // bit as above, instead of being held through the update.
-    if (c.fm) {
+    if (!kCheck && c.fm) {
         float pns[RB];
         if (active) {
             uint32_t qS2[RB];
@@ -913,7 +936,13 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
             copy_rows0(A, next_nn, phi_s + (pslot ^ 1) * Split0::P1_FLOATS + PX + 4, 0,
                        Split0::R1 + 1);
     }
-    if constexpr (CLS == 0) {
+    if constexpr (kCheck) {
+        commit_position(A, n, sm, c, [&]() {
+            if (next_nn >= 0) copy_rows0(A, next_nn, base2 + 4, Split0::P2_R0, WY);
+        });
+        pre = next_nn >= 0 ? 1 : 0;
+        if (pre) pslot ^= 1;
+    } else if constexpr (CLS == 0) {
         reduce_commit<CLS>(A, n, sm, c, sm.csraw[nxt ^ 1], aV, aCx, aCy, aPer,
                            blox, bhix, bloy, bhiy, pmaxb, [&]() {
             if (next_nn >= 0) copy_rows0(A, next_nn, base2 + 4, Split0::P2_R0, WY);
@@ -1066,6 +1095,7 @@ __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
 // -------------------------------------------------------------------------
 // The persistent step kernel (src/kernels.cu:948-1030).
 // -------------------------------------------------------------------------
+template <bool kCheck>
 __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const int* run_if) {
     __shared__ Smem sm;
     const int tid = (int)threadIdx.x;
@@ -1106,7 +1136,9 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         const int n = __builtin_amdgcn_readfirstlane(sm.slot[cur].n);
         if (cls < 0) break;
         switch (cls) {
-            case 0: cell_staged<0>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre, pslot); break;
+            case 0:
+                cell_staged<0, kCheck>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre, pslot);
+                break;
             case 1: cell_cold<1>(&sm.args, n, step, &sm, cur); break;
             case 2: cell_cold<2>(&sm.args, n, step, &sm, cur); break;
             case 3: cell_cold<3>(&sm.args, n, step, &sm, cur); break;
@@ -1398,7 +1430,8 @@ int pfh2d_step_grid(int device) {
 }
 
 namespace {
-int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* stream) {
+int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* stream,
+                bool check = false) {
     if (!a) return invalid("args");
     if (a->moments) return invalid("moments must be NULL");
     if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
@@ -1413,8 +1446,12 @@ int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* str
     // the step addresses S with 32-bit byte offsets
     if (4ull * (unsigned long long)a->P * (unsigned long long)a->L > 0xFFFFFFFFull)
         return invalid("aggregate larger than 4 GiB");
-    hipLaunchKernelGGL(pfh::k_step, dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                       as_stream(stream), *a, run_if);
+    if (check)
+        hipLaunchKernelGGL(pfh::k_step<true>, dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                           as_stream(stream), *a, run_if);
+    else
+        hipLaunchKernelGGL(pfh::k_step<false>, dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                           as_stream(stream), *a, run_if);
     return launch_status("k_step");
 }
 }  // namespace
@@ -1434,15 +1471,21 @@ int pfh2d_launch_step_if(const pfh2d_step_args* a, const int* run_if, int grid,
     return step_launch(a, run_if, grid, stream);
 }
 
+int pfh2d_launch_check_step_if(const pfh2d_step_args* a, const int* run_if, int grid,
+                               void* stream) {
+    if (a && a->S_sc) return invalid("a check step scatters nothing: S_sc must be NULL");
+    return step_launch(a, run_if, grid, stream, true);
+}
+
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out) {
     if (!out) return invalid("out");
     hipFuncAttributes fa{};
-    int st = hip_status(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pfh::k_step)),
+    int st = hip_status(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pfh::k_step<false>)),
                         "hipFuncGetAttributes(k_step)");
     if (st != PFH_OK) return st;
     int blocks = 0;
     st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &blocks, reinterpret_cast<const void*>(pfh::k_step), pfh::kNT, 0),
+                        &blocks, reinterpret_cast<const void*>(pfh::k_step<false>), pfh::kNT, 0),
                     "hipOccupancyMaxActiveBlocksPerMultiprocessor(k_step)");
     if (st != PFH_OK) return st;
     int max_threads = 0;

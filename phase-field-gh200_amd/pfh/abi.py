@@ -154,6 +154,7 @@ SIGNATURES = {
     "pfh2d_step_grid": (_I, [_I]),
     "pfh2d_launch_step": (_I, [C.POINTER(StepArgs), _I, _P, _P, _SZ, _F, _I]),
     "pfh2d_launch_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
+    "pfh2d_launch_check_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
     "pfh2d_query_step_stats": (_I, [_I, C.POINTER(KernelStats)]),
     "pfh2d_init_tiles": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _F, _F, _P]),
     "pfh2d_init_moments": (_I, [_P, _P, _P, _I, _P]),

@@ -423,7 +423,7 @@ class SurrogateRunner:
         infer_cells(self.net, sim, self.ws, W4=self.W4d, b4=self.b4d, dphi=False, stream=sp,
                     go=self.ctl)
         a = self._check_args(s)
-        abi.check(lib.pfh2d_launch_step_if(C.byref(a), go, sim.grid, sp), "check step")
+        abi.check(lib.pfh2d_launch_check_step_if(C.byref(a), go, sim.grid, sp), "check step")
         abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
                                     self.tol, _ptr(self.err), rej, _ptr(self.counters), go, sp),

@@ -432,3 +432,54 @@ def test_abort_inside_chunk():
     assert sa.steps_done == sb.steps_done == 12 + 32
     for x, y in zip(_state(sa), _state(sb)):
         assert torch.equal(x, y)
+
+
+def test_check_step_equals_gated_plain_step():
+    """The surrogate's check step (pfh2d_launch_check_step_if: no moments,
+    scatter, support box or flags) writes the same new field, window
+    positions and classes, bit for bit, as the full step on the same
+    private copies, on the default grid and on two workgroups."""
+    import ctypes as C
+    from pfh import abi
+    wl = params.workload_c1()
+    sim = sim2d.Sim2D(wl)
+    sim.step(20)
+    runner = su.SurrogateRunner(sim, _random_net(3), K=8, tol=1e30, graphs=False)
+    lib, sp, N = abi.load(), sim._sp(), sim.N
+    s = sim.steps_done
+    go = su._ptr(runner.ctl)
+    now = su._ptr(sim.stepc) + 8 * (s % 2)
+    abi.check(lib.pfh_sur_prep(go, go + 4, su._ptr(sim.cell), su._ptr(runner.cell_chk),
+                               su._ptr(sim.cls), su._ptr(runner.cls_chk), N, None, 0,
+                               su._ptr(runner.cur_chk), su._ptr(runner.step_chk), now, sp),
+              "prep")
+    torch.cuda.synchronize()
+    priv = [runner.cell_chk, runner.cls_chk, runner.cur_chk, runner.step_chk, runner.flags_chk]
+    start = [t.clone() for t in priv]
+    for grid in (sim.grid, 2):
+        outs = []
+        for check in (False, True):
+            for t, t0 in zip(priv, start):
+                t.copy_(t0)
+            runner.phi_chk.zero_()
+            a = runner._check_args(s)
+            say(f"{'check' if check else 'full'} step from step {s} on {grid} workgroups")
+            if check:
+                abi.check(lib.pfh2d_launch_check_step_if(C.byref(a), go, grid, sp), "check")
+            else:
+                abi.check(lib.pfh2d_launch_step_if(C.byref(a), go, grid, sp), "full")
+            torch.cuda.synchronize()
+            cells = runner.cell_chk.cpu().numpy().view(abi.CELL_DTYPE)
+            outs.append((runner.phi_chk.clone(), cells[["gx0", "gy0", "cls"]].copy(),
+                         runner.cls_chk.clone(), runner.cur_chk.clone(),
+                         runner.step_chk.clone()))
+        (pa, ca, la, ua, sa), (pb, cb, lb, ub, sb) = outs
+        assert torch.equal(pa.view(torch.int32), pb.view(torch.int32))
+        assert np.array_equal(ca, cb)
+        assert torch.equal(la, lb)
+        assert torch.equal(ua, ub) and torch.equal(sa, sb)
+        assert float(pa.abs().max()) > 0.9
+    # a check step refuses an aggregate to scatter into
+    a = runner._check_args(s)
+    a.S_sc = sim._s_ptr(0)
+    assert lib.pfh2d_launch_check_step_if(C.byref(a), go, sim.grid, sp) != 0
