@@ -358,7 +358,7 @@ struct InferLDS {
     bf16_t h2[kRing * kRowH];
     bf16_t x[kRing * kRowX];
     float p[kRing * kRowP];
-    float bias[3 * kCH];
+    alignas(16) float bias[3 * kCH];
     CropCtx ctx;
 };
 
@@ -410,30 +410,57 @@ __device__ __forceinline__ int perm_row(int m) {
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-// bias + ReLU + bf16 of a lane's 16 channels, packed as two 8-channel chunks
-__device__ __forceinline__ void relu_pack(const f32x16& acc, const float* bias, u32x4* lo,
-                                          u32x4* hi) {
+// A lane's 16 biases (channels 16 h .. 16 h + 15) as the starting value of
+// its accumulator: the convolution sums its products onto the bias
+__device__ __forceinline__ f32x16 bias_acc(const float* bias) {
+    f32x16 a;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const fx4 v = *reinterpret_cast<const fx4*>(bias + 4 * k);
+        a[4 * k] = v[0];
+        a[4 * k + 1] = v[1];
+        a[4 * k + 2] = v[2];
+        a[4 * k + 3] = v[3];
+    }
+    return a;
+}
+
+// ReLU + bf16 of a lane's 16 channels, packed as two 8-channel chunks
+__device__ __forceinline__ void relu_pack(const f32x16& acc, u32x4* lo, u32x4* hi) {
     uint32_t w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        w[k] = pack_bf16(fmaxf(acc[2 * k] + bias[2 * k], 0.0f),
-                         fmaxf(acc[2 * k + 1] + bias[2 * k + 1], 0.0f));
+        w[k] = pack_bf16(fmaxf(acc[2 * k], 0.0f), fmaxf(acc[2 * k + 1], 0.0f));
     *lo = u32x4{w[0], w[1], w[2], w[3]};
     *hi = u32x4{w[4], w[5], w[6], w[7]};
 }
 
-// 32 -> 32 convolution of one 32-pixel tile of row y from ring `src`
+// 32 -> 32 convolution of one 32-pixel tile of row y from ring `src`, summed
+// onto `acc`. The activation fragments are read kPre k-steps ahead of their
+// MFMA, so each MFMA finds its operand in registers instead of waiting out
+// an LDS round trip. The scheduling groups pin that order (mask 0x100: LDS
+// reads, 0x008: MFMA); the first group also holds the caller's four reads
+// of the bias into `acc`.
+constexpr int kPre = 4;
 __device__ __forceinline__ f32x16 conv32_tile(const bf16_t* src, const bf16x8* wa, int y,
-                                              int tile, int col, int half) {
+                                              int tile, int col, int half, f32x16 acc) {
     const bf16_t* base = src + (32 * tile + col) * kHP + 8 * half;
     const int r0 = ring(y - 1) * kRowH, r1 = ring(y) * kRowH, r2 = ring(y + 1) * kRowH;
-    f32x16 acc = {};
-#pragma unroll
-    for (int s = 0; s < 18; ++s) {
+    auto frag = [&](int s) {
         const int t = s >> 1, ky = t / 3, kx = t - 3 * ky;
         const int ro = ky == 0 ? r0 : ky == 1 ? r1 : r2;
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(base + ro + kx * kHP + 16 * (s & 1));
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], b, acc, 0, 0, 0);
+        return *reinterpret_cast<const bf16x8*>(base + ro + kx * kHP + 16 * (s & 1));
+    };
+    bf16x8 b[kPre];
+#pragma unroll
+    for (int s = 0; s < kPre; ++s) b[s] = frag(s);
+    __builtin_amdgcn_sched_group_barrier(0x100, kPre + 4, 0);
+#pragma unroll
+    for (int s = 0; s < 18; ++s) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], b[s % kPre], acc, 0, 0, 0);
+        if (s + kPre < 18) b[s % kPre] = frag(s + kPre);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (s + kPre < 18) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
     return acc;
 }
@@ -443,10 +470,9 @@ __device__ __forceinline__ f32x16 conv32_tile(const bf16_t* src, const bf16x8* w
 // channels each); the taps are compile-time constants per half, the three
 // ring rows are wave-uniform.
 __device__ __forceinline__ f32x16 conv4_tile(const bf16_t* xr, const bf16x8* wa, int y, int tile,
-                                             int col, int half) {
+                                             int col, int half, f32x16 acc) {
     const int rb0 = ring(y - 1) * kRowX, rb1 = ring(y) * kRowX, rb2 = ring(y + 1) * kRowX;
     const bf16_t* base = xr + (32 * tile + col) * kXP;
-    f32x16 acc = {};
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         uint32_t v[4] = {0u, 0u, 0u, 0u};
@@ -663,8 +689,9 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
                 const int y1 = 2 * s + jrow;
                 u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
                 if (in_crop(y1)) {
-                    const f32x16 a = conv4_tile(sm.x, wx, y1, jtile, col, half);
-                    relu_pack(a, sm.bias + 16 * half, &lo, &hi);
+                    const f32x16 a = conv4_tile(sm.x, wx, y1, jtile, col, half,
+                                                bias_acc(sm.bias + 16 * half));
+                    relu_pack(a, &lo, &hi);
                 }
                 store_h(sm.h1, y1, jtile, col, half, lo, hi);
                 // conv2: h2 row 2s - 3 + jrow
@@ -672,8 +699,9 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
                 lo = u32x4{0u, 0u, 0u, 0u};
                 hi = lo;
                 if (in_crop(y2)) {
-                    const f32x16 a = conv32_tile(sm.h1, wa, y2, jtile, col, half);
-                    relu_pack(a, sm.bias + kCH + 16 * half, &lo, &hi);
+                    const f32x16 a = conv32_tile(sm.h1, wa, y2, jtile, col, half,
+                                                 bias_acc(sm.bias + kCH + 16 * half));
+                    relu_pack(a, &lo, &hi);
                 }
                 store_h(sm.h2, y2, jtile, col, half, lo, hi);
                 // output rows [2s-9, 2s-7): b4 + partials in tap order
@@ -696,9 +724,10 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
                 const int y3 = 2 * s - 6 + jrow;
                 f32x16 pacc = {};
                 if (in_crop(y3)) {
-                    const f32x16 a = conv32_tile(sm.h2, wa, y3, jtile, col, half);
+                    const f32x16 a = conv32_tile(sm.h2, wa, y3, jtile, col, half,
+                                                 bias_acc(sm.bias + 2 * kCH + 16 * half));
                     u32x4 lo, hi;
-                    relu_pack(a, sm.bias + 2 * kCH + 16 * half, &lo, &hi);
+                    relu_pack(a, &lo, &hi);
                     pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[0], as_bf16x8(lo), pacc,
                                                                    0, 0, 0);
                     pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[1], as_bf16x8(lo), pacc,
