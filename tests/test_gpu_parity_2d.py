@@ -231,6 +231,32 @@ def test_c2_parity_and_S_exactness(sim_mod, oracle):
     assert not sim.fatal(), sim.flag_counts()
 
 
+def test_c2_long_run_S_exact(sim_mod, oracle):
+    """1002 graph-replayed steps of the 2048^2 / 1024-cell monolayer: the
+    aggregate S still equals the oracle's scatter of the stored fields bit for
+    bit (no drift from the incremental scatter and clear-ahead rotation), no
+    fatal flag is raised and the field stays finite and bounded."""
+    wl = params.workload_c2()
+    sim = sim_mod.Sim2D(wl)
+    sim.capture_graph()
+    say("C2: 1002 steps from six-step graphs")
+    sim.run(1002)
+    sim.sync()
+    assert sim.steps_done == 1002
+    assert not sim.fatal(), sim.flag_counts()
+    phi = np.ascontiguousarray(sim.phi_now())
+    assert np.isfinite(phi).all()
+    assert phi.max() <= 1.5 and phi.min() >= -0.5 and phi.max() > 0.9
+    lib = oracle.load()
+    cells = sim.cells()          # bound: the oracle reads through their addresses
+    cls = sim.classes()
+    S = np.zeros(sim.words, np.uint32)
+    flags = np.zeros(8, np.uint32)
+    lib.pfo2d_scatter_all(phi.ctypes.data, cells.ctypes.data, cls.ctypes.data, S.ctypes.data,
+                          sim.N, sim.L, sim.P, flags.ctypes.data)
+    assert np.array_equal(S, sim.S_now())
+
+
 def test_graph_replay_equals_launches(sim_mod):
     wl = params.workload_c1()
     say("graph replay vs single launches, 24 steps")
