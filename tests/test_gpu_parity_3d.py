@@ -129,6 +129,31 @@ def test_c3_steps_bit_exact(sim3, o3):
     assert not sim.fatal()
 
 
+def test_c3_long_run_S_exact(sim3, o3):
+    """300 graph-replayed steps of the 256^3 / 128-cell workload (brick 56):
+    the aggregate S equals the oracle's scatter of the stored fields at the
+    stored brick origins bit for bit (no drift over many recentring shifts),
+    no fatal flag, and the field stays finite and bounded."""
+    wl = params3d.workload_c3()
+    sim = sim3.Sim3D(wl)
+    sim.capture_graph()
+    say(f"{wl.name}: 300 steps from graphs")
+    sim.run(300)
+    sim.sync()
+    assert not sim.fatal(), sim.flag_counts()
+    phi = np.ascontiguousarray(sim.phi_now())
+    assert np.isfinite(phi).all()
+    assert phi.max() <= 1.5 and phi.min() >= -0.5 and phi.max() > 0.9
+    cells = sim.cells()          # bound: the oracle reads through their addresses
+    S = np.zeros(sim.words, np.uint32)
+    flags = np.zeros(16, np.uint32)
+    o3.load().pfo3d_scatter(phi.ctypes.data, cells.ctypes.data, S.ctypes.data,
+                            flags.ctypes.data, sim.N, sim.B, sim.L, sim.P)
+    assert not flags.any()
+    assert np.array_equal(S, sim.S_now())
+    assert cells["shift_ctr"].sum() > 0
+
+
 def test_graph_replay_matches_eager(sim3):
     wl = params3d.workload_small()
     say(f"{wl.name} graph replay vs eager")
