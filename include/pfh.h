@@ -530,6 +530,15 @@ int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,
 int pfh_sur_tumble(pfh_cell_state* cells, int N, const long long* step0, int K,
                    unsigned long long polarity_seed, double p_tumble, const int* skip_if,
                    void* stream);
+/* The end of a block in one launch (no reference counterpart): pfh_sur_tumble
+ * with skip_if = rejected, taking the block's first step from step_first (the
+ * check step's private counter, step_chk[0] of pfh_sur_prep), and
+ * pfh_sur_advance. One launch fewer per block than the two calls; the same
+ * effect, since the draws do not read stepc. */
+int pfh_sur_finish(pfh_cell_state* cells, int N, const long long* step_first, int K,
+                   unsigned long long polarity_seed, double p_tumble, long long* stepc,
+                   unsigned long long* cursor, int pin, int* go, const int* rejected,
+                   long long* abort_step, uint32_t* rejected_blocks, void* stream);
 
 #ifdef __cplusplus
 }

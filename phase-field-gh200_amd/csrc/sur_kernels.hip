@@ -1126,13 +1126,8 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
 
 // The polarity of K skipped steps: the plain step's tumble draw
 // (src/kernels.cu:98-114) for steps step0 .. step0+K-1, in order.
-__global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step0_dev, int K,
-                             unsigned long long seed, double p_tumble, const int* skip_if) {
-    if (skip_if && *skip_if) return;
-    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (n >= N) return;
-    const unsigned long long step0 = (unsigned long long)*step0_dev;
-    pfh_cell_state* c = cells + n;
+__device__ void tumble_cell(pfh_cell_state* c, unsigned long long step0, int K,
+                            unsigned long long seed, double p_tumble) {
     float theta = c->theta;
     uint32_t ctr = c->tumble_ctr;
     for (int k = 0; k < K; ++k) {
@@ -1147,6 +1142,14 @@ __global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step
     }
     c->theta = theta;
     c->tumble_ctr = ctr;
+}
+
+__global__ void k_sur_tumble(pfh_cell_state* cells, int N, const long long* step0_dev, int K,
+                             unsigned long long seed, double p_tumble, const int* skip_if) {
+    if (skip_if && *skip_if) return;
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n >= N) return;
+    tumble_cell(cells + n, (unsigned long long)*step0_dev, K, seed, p_tumble);
 }
 
 // Two aggregate slots cleared for an accepted block (nothing when *skip_if).
@@ -1170,10 +1173,9 @@ __global__ void k_sur_zero2(uint32_t* a, uint32_t* b, size_t words, const int* s
 // cursors restart. Rejected while running: *go drops to 0 (every later
 // gated launch does nothing), abort_step records the block's first step and
 // rejected_blocks counts it; the caller then runs the plain steps itself.
-__global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin,
-                              int* go, const int* rejected, long long* abort_step,
-                              uint32_t* rejected_blocks) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void advance_counters(long long* stepc, unsigned long long* cursor, int K, int pin,
+                                 int* go, const int* rejected, long long* abort_step,
+                                 uint32_t* rejected_blocks) {
     if (*rejected) {
         if (*go) {
             *go = 0;
@@ -1186,6 +1188,28 @@ __global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int 
     stepc[1] += K;
     cursor[0] = 0ull;
     cursor[1] = 0ull;
+}
+
+__global__ void k_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin,
+                              int* go, const int* rejected, long long* abort_step,
+                              uint32_t* rejected_blocks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    advance_counters(stepc, cursor, K, pin, go, rejected, abort_step, rejected_blocks);
+}
+
+// The end of a block in one launch: k_sur_tumble's draws (nothing when the
+// block has rejected cells) and k_sur_advance (thread 0 of workgroup 0).
+// The draws take the block's first step from step_first (the check step's
+// private counter, which prep set and the advance does not write), so they
+// do not race with the advance of stepc.
+__global__ void k_sur_finish(pfh_cell_state* cells, int N, const long long* step_first, int K,
+                             unsigned long long seed, double p_tumble, long long* stepc,
+                             unsigned long long* cursor, int pin, int* go, const int* rejected,
+                             long long* abort_step, uint32_t* rejected_blocks) {
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (*rejected == 0 && n < N)
+        tumble_cell(cells + n, (unsigned long long)*step_first, K, seed, p_tumble);
+    if (n == 0) advance_counters(stepc, cursor, K, pin, go, rejected, abort_step, rejected_blocks);
 }
 
 // Start of a block (nothing when *go == 0): the check step's private copies
@@ -1477,6 +1501,19 @@ int pfh_sur_advance(long long* stepc, unsigned long long* cursor, int K, int pin
     hipLaunchKernelGGL(sur::k_sur_advance, dim3(1), dim3(64), 0, as_stream(stream), stepc, cursor,
                        K, pin, go, rejected, abort_step, rejected_blocks);
     return launch_status("k_sur_advance");
+}
+
+int pfh_sur_finish(pfh_cell_state* cells, int N, const long long* step_first, int K,
+                   unsigned long long polarity_seed, double p_tumble, long long* stepc,
+                   unsigned long long* cursor, int pin, int* go, const int* rejected,
+                   long long* abort_step, uint32_t* rejected_blocks, void* stream) {
+    if (!cells || !step_first || !stepc || !cursor || !go || !rejected || !abort_step ||
+        N <= 0 || K <= 0 || (pin & ~1))
+        return invalid("sur_finish");
+    hipLaunchKernelGGL(sur::k_sur_finish, dim3((unsigned)((N + 127) / 128)), dim3(128), 0,
+                       as_stream(stream), cells, N, step_first, K, polarity_seed, p_tumble,
+                       stepc, cursor, pin, go, rejected, abort_step, rejected_blocks);
+    return launch_status("k_sur_finish");
 }
 
 int pfh_sur_prep(const int* go, int* rejected, const pfh_cell_state* cells,

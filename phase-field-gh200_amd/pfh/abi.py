@@ -200,6 +200,8 @@ SIGNATURES = {
     "pfh_sur_rel_l2": (_I, [_P, _P, _P, _I, _P]),
     "pfh_sur_writeback": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P]),
     "pfh_sur_tumble": (_I, [_P, _I, _P, _I, C.c_ulonglong, C.c_double, _P, _P]),
+    "pfh_sur_finish": (_I, [_P, _I, _P, _I, C.c_ulonglong, C.c_double, _P, _P, _I, _P, _P,
+                            _P, _P, _P]),
 }
 
 # Entry points the ABI header declares; tests check the library exports all.

@@ -433,14 +433,15 @@ class SurrogateRunner:
         # ahead by itself
         abi.check(lib.pfh_sur_zero2(sim._s_ptr(e % 3), sim._s_ptr((e + 1) % 3), sim.words, rej,
                                     sp), "zero2")
-        abi.check(lib.pfh_sur_tumble(_ptr(sim.cell), N, now, K, sim.p.polarity_stream(),
-                                     sim.p.p_tumble(), rej, sp), "tumble")
         abi.check(lib.pfh_sur_commit(_ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls),
                                      _ptr(self.ws.origin), _ptr(self.ws.pred), N, sim.L, sim.P,
                                      sim._s_ptr(e % 3), _ptr(sim.flags), rej, sp), "commit")
-        abi.check(lib.pfh_sur_advance(_ptr(sim.stepc), _ptr(sim.cursor), K, pin, go, rej,
-                                      _ptr(self.abort_step), _ptr(self.counters) + 4, sp),
-                  "advance")
+        # the K polarity draws (from the block's first step, which prep left
+        # in step_chk[0]) and the counter advance, in one launch
+        abi.check(lib.pfh_sur_finish(_ptr(sim.cell), N, _ptr(self.step_chk), K,
+                                     sim.p.polarity_stream(), sim.p.p_tumble(), _ptr(sim.stepc),
+                                     _ptr(sim.cursor), pin, go, rej, _ptr(self.abort_step),
+                                     _ptr(self.counters) + 4, sp), "finish")
 
     def _captured(self, r: int) -> torch.cuda.CUDAGraph:
         g = self._g.get(r)

@@ -483,3 +483,39 @@ def test_check_step_equals_gated_plain_step():
     a = runner._check_args(s)
     a.S_sc = sim._s_ptr(0)
     assert lib.pfh2d_launch_check_step_if(C.byref(a), go, sim.grid, sp) != 0
+
+
+def test_block_finish_equals_tumble_and_advance():
+    """pfh_sur_finish (the K polarity draws and the counter advance in one
+    launch) leaves theta, tumble_ctr and the step counters of an accepted
+    block exactly as pfh_sur_tumble applied to the block's starting cell
+    states from its first step, on the 2048^2 / 1024-cell monolayer."""
+    from pfh import abi
+    wl = params.workload_c2()
+    sim = sim2d.Sim2D(wl)
+    sim.step(20)
+    runner = su.SurrogateRunner(sim, _random_net(3), K=8, tol=1e30, graphs=False)
+    lib = abi.load()
+    s = sim.steps_done
+    cell0 = sim.cell.clone()
+    step0 = sim.stepc.clone()
+    say(f"one accepted block from step {s}, draws against pfh_sur_tumble")
+    runner.block()
+    torch.cuda.synchronize()
+    assert runner.stats()["accepted_blocks"] == 1
+    ref = cell0.clone()
+    abi.check(lib.pfh_sur_tumble(ref.data_ptr(), sim.N, step0.data_ptr() + 8 * (s % 2), 8,
+                                 sim.p.polarity_stream(), sim.p.p_tumble(), None, None),
+              "tumble")
+    torch.cuda.synchronize()
+    got = sim.cells()
+    want = ref.cpu().numpy().view(abi.CELL_DTYPE)
+    assert np.array_equal(got["theta"].view(np.uint32), want["theta"].view(np.uint32))
+    assert np.array_equal(got["tumble_ctr"], want["tumble_ctr"])
+    assert torch.equal(sim.stepc, step0 + 8)
+    start = cell0.cpu().numpy().view(abi.CELL_DTYPE)
+    expected = sim.p.p_tumble() * 8 * sim.N
+    say(f"tumbles in the block: {int((got['tumble_ctr'] - start['tumble_ctr']).sum())} "
+        f"(expected about {expected:.1f})")
+    if expected > 5:
+        assert (got["tumble_ctr"] != start["tumble_ctr"]).any()
