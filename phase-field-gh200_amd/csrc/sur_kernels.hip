@@ -551,10 +551,10 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
     // wx: conv1 fragments (waves 0-5: k-steps 0-2) or the output layer's
     // (waves 6-11: hi 0, lo 0, hi 1, lo 1); one array keeps both in one set of
     // registers
-    bf16x8 wx[4];
+    bf16x8 wx[3];
     if (early) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < 3; ++s) {
             uint32_t v[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -572,26 +572,28 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
             wx[s] = as_bf16x8(u32x4{v[0], v[1], v[2], v[3]});
         }
     } else {
+        // Output layer, channels 16 j .. 16 j + 15 per fragment: rows 0-8
+        // hold the bf16 high parts of the nine taps' weights, rows 16-24 the
+        // low parts (w - hi, rounded to bf16), so one MFMA yields both
+        // partial sums of a tap (accumulators r and r + 8 of a lane).
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            uint32_t vh[4], vl[4];
+            uint32_t v[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                uint32_t ph = 0u, pl = 0u;
+                uint32_t pr = 0u;
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
                     const int c = 16 * half + 8 * j + 2 * k + e;
-                    const float w = col < 9 ? W4[col * kCH + c] : 0.0f;
+                    const int t = col < 16 ? col : col - 16;
+                    const float w = t < 9 ? W4[t * kCH + c] : 0.0f;
                     const bf16_t hb = f2bf(w);
-                    const bf16_t lb = f2bf(w - bf2f(hb));
-                    ph |= (uint32_t)hb << (16 * e);
-                    pl |= (uint32_t)lb << (16 * e);
+                    const bf16_t part = col < 16 ? hb : f2bf(w - bf2f(hb));
+                    pr |= (uint32_t)part << (16 * e);
                 }
-                vh[k] = ph;
-                vl[k] = pl;
+                v[k] = pr;
             }
-            wx[2 * j] = as_bf16x8(u32x4{vh[0], vh[1], vh[2], vh[3]});
-            wx[2 * j + 1] = as_bf16x8(u32x4{vl[0], vl[1], vl[2], vl[3]});
+            wx[j] = as_bf16x8(u32x4{v[0], v[1], v[2], v[3]});
         }
     }
     // Retire the fragment loads here: otherwise the counter-based waits in
@@ -738,18 +740,15 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
                     relu_pack(a, &lo, &hi);
                     pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[0], as_bf16x8(lo), pacc,
                                                                    0, 0, 0);
-                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[1], as_bf16x8(lo), pacc,
-                                                                   0, 0, 0);
-                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[2], as_bf16x8(hi), pacc,
-                                                                   0, 0, 0);
-                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[3], as_bf16x8(hi), pacc,
+                    pacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wx[1], as_bf16x8(hi), pacc,
                                                                    0, 0, 0);
                 }
-                // lane (pixel, half 0) holds taps 0-3 and 8, half 1 taps 4-7
+                // lane (pixel, half 0) holds taps 0-3 and 8, half 1 taps 4-7:
+                // high parts in accumulators r, low parts in r + 8
                 float* pr = sm.p + ring(y3) * kRowP + 32 * jtile + col + 1;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) pr[(4 * half + r) * kHaloCols] = pacc[r];
-                if (half == 0) pr[8 * kHaloCols] = pacc[4];
+                for (int r = 0; r < 4; ++r) pr[(4 * half + r) * kHaloCols] = pacc[r] + pacc[r + 8];
+                if (half == 0) pr[8 * kHaloCols] = pacc[4] + pacc[12];
             }
             preg = pnext;
             // the rings are the only data shared between lanes here
