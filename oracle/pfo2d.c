@@ -117,6 +117,23 @@ static double pfo_tree(const double* part, const pfo_layout* l) {
     return total;
 }
 
+/* Run-and-tumble draw of one cell-step (src/kernels.cu:98-112): counter
+ * (step_lo, step_hi, gid, 0xA5A5A5A5), key = polarity stream; a tumble when
+ * u53(v0, v1) < p, then theta = (float)(2 pi u53(v2, v3)). Returns 1 on a
+ * tumble and writes the new angle; otherwise leaves *theta. */
+int pfo2d_tumble(uint64_t step, int32_t gid, uint64_t stream, double p_tumble,
+                 float* theta) {
+    const pfo_philox4 r = pfo_philox4x32_10(
+        (uint32_t)(step & 0xFFFFFFFFull), (uint32_t)(step >> 32),
+        (uint32_t)gid, 0xA5A5A5A5u,
+        (uint32_t)(stream & 0xFFFFFFFFull), (uint32_t)(stream >> 32));
+    if (pfo_uniform53(r.v[0], r.v[1]) < p_tumble) {
+        *theta = (float)(2.0 * PFO_PI * pfo_uniform53(r.v[2], r.v[3]));
+        return 1;
+    }
+    return 0;
+}
+
 /* ---- binary32 and FP64 instantiations of the step ---- */
 #define REAL float
 #define SFX _f32

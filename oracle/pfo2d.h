@@ -134,6 +134,8 @@ void pfo2d_pack_traj(const pfo_cell* cell, const uint8_t* cls, pfo_traj* out,
                      int N, int L);
 
 /* Scalars. */
+int pfo2d_tumble(uint64_t step, int32_t gid, uint64_t stream, double p_tumble,
+                 float* theta);
 void pfo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double pfo_u53(uint32_t a, uint32_t b);
 uint32_t pfo_qof(float phi);
