@@ -174,7 +174,8 @@ int pfh_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* Replaces configure_k_step_smem() (include/kernels.cuh:219): sets the
  * kernel's LDS attribute on the current device. Idempotent. */
 int pfh2d_configure_step(void);
-/* Replaces configure_morton_smem(int) (include/kernels.cuh:220). */
+/* Replaces configure_morton_smem(int) (include/kernels.cuh:220). The sort's
+ * LDS is static (sized for 16384 cells), so this only validates the size. */
 int pfh2d_configure_morton(int smem_bytes);
 /* Replaces k_step_grid(device) (include/kernels.cuh:221): the persistent
  * grid, one 1024-thread workgroup per compute unit. Returns the grid size,
@@ -225,9 +226,14 @@ int pfh2d_init_moments(pfh_cell_state* cell, const uint8_t* cls,
 int pfh2d_scatter_all(const float* phi, const pfh_cell_state* cell,
                       const uint8_t* cls, uint32_t* S, int N, int L, int P,
                       uint32_t* flags, void* stream);
-/* Replaces k_zero_u32 (src/kernels.cu:1044-1048). */
+/* Replaces k_zero_u32 (src/kernels.cu:1044-1048): n words set to 0, any
+ * 4 B alignment. */
 int pfh2d_zero_u32(uint32_t* p, size_t n, void* stream);
-/* Replaces k_verify_cells (src/kernels.cu:1193-1232). */
+/* Replaces k_verify_cells (src/kernels.cu:1193-1232): per cell, out_V[n] =
+ * sum of phi^2 over the active window of class cls[n] (the update's
+ * reduction order, so it is the V the moments pass would store) and
+ * out_outside_max[n] = max |phi| over the rest of the tile. phi 16 B
+ * aligned. cell is not read (kept for the reference's signature). */
 int pfh2d_verify_cells(const float* phi, const pfh_cell_state* cell,
                        const uint8_t* cls, int N, double* out_V,
                        float* out_outside_max, void* stream);
@@ -236,11 +242,13 @@ int pfh2d_verify_cells(const float* phi, const pfh_cell_state* cell,
 int pfh2d_verify_S(const uint32_t* S, size_t n, uint32_t* out_max,
                    void* stream);
 /* Replaces k_pack_traj (src/kernels.cu:1242-1265). out may be device memory
- * or mapped pinned host memory. */
+ * or mapped pinned host memory; cell and out 16 B aligned. */
 int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,
                     pfh_traj_packed_cell* out, int N, int L, void* stream);
-/* Replaces k_morton_sort (src/kernels.cu:1267-1302): M is a power of two
- * >= N, M <= 16384. */
+/* Replaces k_morton_sort (src/kernels.cu:1267-1302): perm = the cells sorted
+ * by (Morton code of the truncated centroid, index). M is a power of two
+ * >= N, M <= 16384 (validated as in the reference; the sort itself does not
+ * pad to M). */
 int pfh2d_morton_sort(const pfh_cell_state* cell, uint32_t* perm, int N,
                       int M, int L, void* stream);
 
@@ -345,11 +353,13 @@ typedef struct pfh3d_step_args {
 _Static_assert(sizeof(pfh3d_cell_state) == 256, "CellState3D is 256 B");
 _Static_assert(offsetof(pfh3d_cell_state, storage_edge) == 192, "storage_edge moved");
 _Static_assert(sizeof(pfh3d_moment_partial) == 96, "MomentPartial3D is 96 B");
+_Static_assert(sizeof(pfh3d_traj_cell) == 104, "TrajPackedCell3D is 104 B");
 _Static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
 #else
 static_assert(sizeof(pfh3d_cell_state) == 256, "CellState3D is 256 B");
 static_assert(offsetof(pfh3d_cell_state, storage_edge) == 192, "storage_edge moved");
 static_assert(sizeof(pfh3d_moment_partial) == 96, "MomentPartial3D is 96 B");
+static_assert(sizeof(pfh3d_traj_cell) == 104, "TrajPackedCell3D is 104 B");
 static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
 #endif
 

@@ -82,6 +82,7 @@ def load() -> C.CDLL:
         "pfo2d_verify_cells": (None, [_P, _P, _P, C.c_int, _P, _P]),
         "pfo2d_verify_S": (C.c_uint32, [_P, C.c_size_t]),
         "pfo2d_pack_traj": (None, [_P, _P, _P, C.c_int, C.c_int]),
+        "pfo2d_morton_perm": (None, [_P, C.c_int, C.c_int, _P]),
         "pfo_philox": (None, [_P, _P, _P]),
         "pfo2d_tumble": (C.c_int, [C.c_uint64, C.c_int32, C.c_uint64, C.c_double, _P]),
         "pfo_u53": (C.c_double, [C.c_uint32, C.c_uint32]),

@@ -251,27 +251,57 @@ void pfo2d_scatter_all_f64(const double* phi, const pfo_cell* cell,
     scatter_all_f64(phi, cell, cls, S, N, L, P, flags);
 }
 
-/* k_verify_cells, src/kernels.cu:1193-1232 (sequential sum). */
+/* k_verify_cells, src/kernels.cu:1193-1232: the window's sum of phi^2 (here
+ * in the update's reduction order, i.e. the V the moments pass stores) and
+ * max |phi| over the tile outside the window. */
 void pfo2d_verify_cells(const float* phi, const pfo_cell* cell,
                         const uint8_t* clsv, int N, double* out_V,
                         float* out_outside_max) {
-    (void)cell;
+    pfo_cell* tmp = (pfo_cell*)calloc((size_t)(N > 0 ? N : 1), sizeof(pfo_cell));
+    if (N > 0) memcpy(tmp, cell, sizeof(pfo_cell) * (size_t)N);
+    init_moments_f32(tmp, clsv, phi, N, PFO_ORDER_MI355X);
     for (int n = 0; n < N; ++n) {
         const int c = (int)clsv[n];
         const int wx = kCls[c].wx, wy = kCls[c].wy;
         const int tx0 = kCls[c].tx0, ty0 = kCls[c].ty0;
         const float* t = phi + (size_t)n * PFO_TILE_AREA;
-        double v = 0.0;
         float omax = 0.0f;
         for (int i = 0; i < PFO_TILE_AREA; ++i) {
             const int x = i % PFO_TILE_PITCH, y = i / PFO_TILE_PITCH;
             const int inside = x >= tx0 && x < tx0 + wx && y >= ty0 && y < ty0 + wy;
-            if (inside) v += (double)t[i] * (double)t[i];
-            else omax = fmaxf(omax, fabsf(t[i]));
+            if (!inside) omax = fmaxf(omax, fabsf(t[i]));
         }
-        out_V[n] = v;
+        out_V[n] = tmp[n].V;
         out_outside_max[n] = omax;
     }
+    free(tmp);
+}
+
+/* Morton queue order (k_morton_sort, src/kernels.cu:1267-1302): cells sorted
+ * by the 64-bit key (Morton code of the truncated, wrapped centroid << 32 |
+ * index); the Morton code interleaves x into the even and y into the odd
+ * bits of 16-bit coordinates. */
+static uint32_t pfo_part1by1(uint32_t v) {
+    uint32_t r = 0u;
+    for (int i = 0; i < 16; ++i) r |= ((v >> i) & 1u) << (2 * i);
+    return r;
+}
+static int pfo_cmp_u64(const void* a, const void* b) {
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+void pfo2d_morton_perm(const pfo_cell* cell, int N, int L, uint32_t* perm) {
+    uint64_t* k = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(N > 0 ? N : 1));
+    for (int i = 0; i < N; ++i) {
+        const double V = cell[i].V > 0.0 ? cell[i].V : 1.0;
+        const uint32_t x = (uint32_t)pfo_wrapi(cell[i].gx0 + (int)(cell[i].Cx / V), L);
+        const uint32_t y = (uint32_t)pfo_wrapi(cell[i].gy0 + (int)(cell[i].Cy / V), L);
+        const uint32_t m = pfo_part1by1(x & 0xFFFFu) | (pfo_part1by1(y & 0xFFFFu) << 1);
+        k[i] = ((uint64_t)m << 32) | (uint32_t)i;
+    }
+    qsort(k, (size_t)N, sizeof(uint64_t), pfo_cmp_u64);
+    for (int i = 0; i < N; ++i) perm[i] = (uint32_t)(k[i] & 0xFFFFFFFFull);
+    free(k);
 }
 
 uint32_t pfo2d_verify_S(const uint32_t* S, size_t n) {

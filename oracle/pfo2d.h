@@ -130,6 +130,7 @@ void pfo2d_verify_cells(const float* phi, const pfo_cell* cell,
                         const uint8_t* cls, int N, double* out_V,
                         float* out_outside_max);
 uint32_t pfo2d_verify_S(const uint32_t* S, size_t n);
+void pfo2d_morton_perm(const pfo_cell* cell, int N, int L, uint32_t* perm);
 void pfo2d_pack_traj(const pfo_cell* cell, const uint8_t* cls, pfo_traj* out,
                      int N, int L);
 

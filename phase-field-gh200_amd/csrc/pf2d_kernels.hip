@@ -1161,44 +1161,91 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
 
 // -------------------------------------------------------------------------
 // Initial condition, verification, trajectory, ordering.
+//
+// Layout shared by these kernels: a tile row is 72 groups of 16 B; work is
+// spread over (cell, band of rows) workgroups so that a small population
+// still fills the chip, every global access is a 16 B vector or a lane per
+// word of one row, and flag words take at most one atomic per wave.
 // -------------------------------------------------------------------------
-__global__ void k_zero_u32(uint32_t* p, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
+constexpr int kRowGroups = kTilePitch / 4;          // 16 B groups per tile row
+constexpr int kInitBandRows = 32;                   // 288 = 9 bands
+constexpr int kInitBands = kTilePitch / kInitBandRows;
+constexpr int kScatterBandRows = 16;
+constexpr int kScatterBands = (286 + kScatterBandRows - 1) / kScatterBandRows;
+
+__device__ __forceinline__ float f4_get(const float4& v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
-__global__ void k_init_tiles(float* phi_a, float* phi_b, pfh_cell_state* cell,
-                             const uint8_t* clsv, int N, int L,
-                             const float* seed_cx, const float* seed_cy,
-                             float radius_eff, float kappa_iface) {
-    const int n = blockIdx.x;
-    if (n >= N) return;
+// Fill n words with v: 16 B stores on the aligned body, words at both ends.
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t head = (size_t)((0u - (uint32_t)(uintptr_t)p) & 15u) >> 2;
+    if (head > n) head = n;
+    if (tid < head) p[tid] = v;
+    u32x4* body = reinterpret_cast<u32x4*>(p + head);
+    const size_t nv = (n - head) >> 2;
+    const u32x4 v4 = {v, v, v, v};
+    for (size_t i = tid; i < nv; i += stride) body[i] = v4;
+    const size_t done = head + 4 * nv;
+    if (tid < n - done) p[done + tid] = v;
+}
+
+// Smoothed disk of one window pixel (k_init_tiles' profile,
+// src/kernels.cu:1072-1085): binary32 distance with minimum-image wrap,
+// 0.5 (1 - tanh(k (r - R_eff))).
+__device__ __forceinline__ float disk_value(int gx0, int gy0, int a, int b, float cx,
+                                            float cy, int L, float radius_eff,
+                                            float kappa_iface) {
+    const float half = 0.5f * (float)L;
+    float dx = (float)(gx0 + a) - cx;
+    float dy = (float)(gy0 + b) - cy;
+    if (dx > half) dx -= (float)L;
+    if (dx < -half) dx += (float)L;
+    if (dy > half) dy -= (float)L;
+    if (dy < -half) dy += (float)L;
+    const float r = sqrtf(dx * dx + dy * dy);
+    const float t = (float)tanh_d((double)(kappa_iface * (r - radius_eff)));
+    return 0.5f * (1.0f - t);
+}
+
+// grid (N, 9): workgroup (n, band) writes tile rows band*32 .. band*32+31 of
+// both tiles of cell n in one pass of 16 B stores: the disk inside the class
+// window, zeros elsewhere (phi_b all zeros). Band 0 records the origin.
+__global__ __launch_bounds__(256) void k_init_tiles(float* phi_a, float* phi_b,
+                                                    pfh_cell_state* cell,
+                                                    const uint8_t* clsv, int L,
+                                                    const float* seed_cx,
+                                                    const float* seed_cy,
+                                                    float radius_eff, float kappa_iface) {
+    const int n = (int)blockIdx.x;
     const int cls = (int)clsv[n];
     const Shape sc = shape_of(cls);
-    float* ta = phi_a + (size_t)n * kTileArea;
-    float* tb = phi_b + (size_t)n * kTileArea;
-    for (int i = (int)threadIdx.x; i < kTileArea; i += (int)blockDim.x) {
-        ta[i] = 0.0f;
-        tb[i] = 0.0f;
-    }
-    __syncthreads();
     const float cx = seed_cx[n], cy = seed_cy[n];
     const int gx0 = wrapi((int)rintf(cx - 0.5f * (float)(sc.wx - 1)), L);
     const int gy0 = wrapi((int)rintf(cy - 0.5f * (float)(sc.wy - 1)), L);
-    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += (int)blockDim.x) {
-        const int a = p % sc.wx, b = p / sc.wx;
-        float dx = (float)(gx0 + a) - cx;
-        float dy = (float)(gy0 + b) - cy;
-        if (dx > 0.5f * (float)L) dx -= (float)L;
-        if (dx < -0.5f * (float)L) dx += (float)L;
-        if (dy > 0.5f * (float)L) dy -= (float)L;
-        if (dy < -0.5f * (float)L) dy += (float)L;
-        const float r = sqrtf(dx * dx + dy * dy);
-        const float t = (float)tanh_d((double)(kappa_iface * (r - radius_eff)));
-        ta[(size_t)(sc.ty0 + b) * kTilePitch + sc.tx0 + a] = 0.5f * (1.0f - t);
+    float4* ta = reinterpret_cast<float4*>(phi_a + (size_t)n * kTileArea);
+    float4* tb = reinterpret_cast<float4*>(phi_b + (size_t)n * kTileArea);
+    const int row0 = (int)blockIdx.y * kInitBandRows;
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int g = (int)threadIdx.x; g < kInitBandRows * kRowGroups; g += (int)blockDim.x) {
+        const int ty = row0 + g / kRowGroups, gx = g % kRowGroups;
+        const int b = ty - sc.ty0;
+        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if ((unsigned)b < (unsigned)sc.wy) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int a = 4 * gx + k - sc.tx0;
+                if ((unsigned)a < (unsigned)sc.wx)
+                    v[k] = disk_value(gx0, gy0, a, b, cx, cy, L, radius_eff, kappa_iface);
+            }
+        }
+        const size_t o = (size_t)ty * kRowGroups + gx;
+        ta[o] = make_float4(v[0], v[1], v[2], v[3]);
+        tb[o] = zero4;
     }
-    if (threadIdx.x == 0) {
+    if (blockIdx.y == 0 && threadIdx.x == 0) {
         pfh_cell_state* cs = &cell[n];
         cs->gx0 = gx0;
         cs->gy0 = gy0;
@@ -1223,62 +1270,80 @@ __global__ __launch_bounds__(kNT) void k_init_moments(pfh_cell_state* cell,
     if (threadIdx.x == 0) store_moments(&cell[n], m);
 }
 
-__global__ void k_scatter_all(const float* phi, const pfh_cell_state* cell,
-                              const uint8_t* clsv, uint32_t* S, int N, int L,
-                              int P, uint32_t* flags) {
-    const int n = blockIdx.x;
-    if (n >= N) return;
-    const int cls = (int)clsv[n];
-    const Shape sc = shape_of(cls);
+// grid (N, 18): workgroup (n, band) adds q(phi) of window rows band*16 ..
+// band*16+15 of cell n into S; wave w takes rows w, w+4, ..., its lanes one
+// word of the row each (coalesced loads, atomics only where q != 0). Flag
+// conditions are collected per lane and raised by one atomic per wave.
+__global__ __launch_bounds__(256) void k_scatter_all(const float* phi,
+                                                     const pfh_cell_state* cell,
+                                                     const uint8_t* clsv, uint32_t* S,
+                                                     int L, int P, uint32_t* flags) {
+    const int n = (int)blockIdx.x;
+    const Shape sc = shape_of((int)clsv[n]);
+    const int r0 = (int)blockIdx.y * kScatterBandRows;
+    if (r0 >= sc.wy) return;
+    const int r1 = min(r0 + kScatterBandRows, sc.wy);
     const int gx0 = cell[n].gx0, gy0 = cell[n].gy0;
-    const float* t = phi + (size_t)n * kTileArea;
-    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += (int)blockDim.x) {
-        const int a = p % sc.wx, b = p / sc.wx;
-        const float pn = t[(size_t)(sc.ty0 + b) * kTilePitch + sc.tx0 + a];
-        const uint32_t q = q_of(pn);
-        if (!q) continue;
-        const uint32_t old = atomicAdd(
-            &S[(size_t)wrapi(gy0 + b, L) * P + wrapi(gx0 + a, L)], q);
-        if (old > 0xFFFFFFFFu - q) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
-        if (pn * pn > kQClampPhiSq) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
+    const int lane = (int)threadIdx.x & (kWave - 1), wave = (int)threadIdx.x / kWave;
+    const float* win = phi + (size_t)n * kTileArea + (size_t)sc.ty0 * kTilePitch + sc.tx0;
+    uint32_t overflow = 0u, clamp = 0u;
+    for (int b = r0 + wave; b < r1; b += (int)blockDim.x / kWave) {
+        uint32_t* srow = S + (size_t)wrapi(gy0 + b, L) * (size_t)P;
+        const float* prow = win + (size_t)b * kTilePitch;
+        for (int a = lane; a < sc.wx; a += kWave) {
+            const float pn = prow[a];
+            const uint32_t q = q_of(pn);
+            if (q) {
+                const uint32_t old = __hip_atomic_fetch_add(&srow[wrapi(gx0 + a, L)], q,
+                                                            __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+                overflow |= old > 0xFFFFFFFFu - q;
+                clamp |= pn * pn > kQClampPhiSq;
+            }
+        }
+    }
+    const bool any_overflow = __ballot(overflow) != 0ull;
+    const bool any_clamp = __ballot(clamp) != 0ull;
+    if (lane == 0) {
+        if (any_overflow) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
+        if (any_clamp) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
     }
 }
 
-__global__ __launch_bounds__(1024) void k_verify_cells(
-    const float* phi, const pfh_cell_state* cell, const uint8_t* clsv, int N,
-    double* out_V, float* out_outside_max) {
-    __shared__ double vred[kWaves];
-    __shared__ float ored[kWaves];
-    const int n = blockIdx.x;
-    if (n >= N) return;
-    const Shape sc = shape_of((int)clsv[n]);
-    const float* t = phi + (size_t)n * kTileArea;
-    double v = 0.0;
+// Per cell: V of the stored window in the update's reduction order (the
+// moments pass), and max |phi| over the tile outside the window (a 16 B sweep
+// of the whole tile). Sim::verify compares V with CellState.V.
+__global__ __launch_bounds__(kNT) void k_verify_cells(const float* phi,
+                                                      const uint8_t* clsv,
+                                                      double* out_V,
+                                                      float* out_outside_max) {
+    __shared__ float wmax[kWaves];
+    const int n = (int)blockIdx.x;
+    const int cls = (int)clsv[n];
+    const Shape sc = shape_of(cls);
+    const float* tile = phi + (size_t)n * kTileArea;
+    const float4* t4 = reinterpret_cast<const float4*>(tile);
     float omax = 0.0f;
-    for (int i = (int)threadIdx.x; i < kTileArea; i += (int)blockDim.x) {
-        const int x = i % kTilePitch, y = i / kTilePitch;
-        const float p = t[i];
-        const bool inside = x >= sc.tx0 && x < sc.tx0 + sc.wx &&
-                            y >= sc.ty0 && y < sc.ty0 + sc.wy;
-        if (inside) v += (double)p * (double)p;
-        else omax = fmaxf(omax, fabsf(p));
-    }
-    v = wave_sum(v);
+    for (int g = (int)threadIdx.x; g < kTileArea / 4; g += kNT) {
+        const int y = g / kRowGroups, x0 = 4 * (g % kRowGroups);
+        const float4 v = t4[g];
+        const bool row_in = (unsigned)(y - sc.ty0) < (unsigned)sc.wy;
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1)
-        omax = fmaxf(omax, __shfl_down(omax, d, kWave));
-    const int lane = (int)threadIdx.x & (kWave - 1), wave = (int)threadIdx.x / kWave;
-    if (lane == 0) { vred[wave] = v; ored[wave] = omax; }
-    __syncthreads();
+        for (int k = 0; k < 4; ++k)
+            if (!(row_in && (unsigned)(x0 + k - sc.tx0) < (unsigned)sc.wx))
+                omax = fmaxf(omax, fabsf(f4_get(v, k)));
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) omax = fmaxf(omax, __shfl_xor(omax, d, kWave));
+    if (((int)threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = omax;
+    const float* win = tile + (size_t)sc.ty0 * kTilePitch + sc.tx0;
+    const WindowMoments m =
+        window_moments(cls, [&](int a, int b) { return win[(size_t)b * kTilePitch + a]; });
     if (threadIdx.x == 0) {
-        double sv = 0.0;
-        float so = 0.0f;
-        for (int w = 0; w < (int)blockDim.x / kWave; ++w) {
-            sv += vred[w];
-            so = fmaxf(so, ored[w]);
-        }
-        out_V[n] = sv;
-        out_outside_max[n] = so;
+        float o = 0.0f;
+        for (int w = 0; w < kWaves; ++w) o = fmaxf(o, wmax[w]);
+        out_V[n] = m.V;
+        out_outside_max[n] = o;
     }
 }
 
@@ -1291,84 +1356,183 @@ __global__ __launch_bounds__(256) void k_stream_copy(u32x4* dst, const u32x4* sr
         dst[i] = src[i];
 }
 
-__global__ void k_verify_S(const uint32_t* S, size_t n, uint32_t* out_max) {
-    uint32_t m = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        m = max(m, S[i]);
-    atomicMax(out_max, m);
+// Max word of S into *out_max: 16 B loads on the aligned body, a per-wave
+// maximum, then one atomic per wave that saw a nonzero word.
+__global__ __launch_bounds__(256) void k_verify_S(const uint32_t* S, size_t n,
+                                                  uint32_t* out_max) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t head = (size_t)((0u - (uint32_t)(uintptr_t)S) & 15u) >> 2;
+    if (head > n) head = n;
+    uint32_t m = tid < head ? S[tid] : 0u;
+    const u32x4* body = reinterpret_cast<const u32x4*>(S + head);
+    const size_t nv = (n - head) >> 2;
+    for (size_t i = tid; i < nv; i += stride) {
+        const u32x4 v = body[i];
+        m = max(m, max(max(v.x, v.y), max(v.z, v.w)));
+    }
+    const size_t done = head + 4 * nv;
+    if (tid < n - done) m = max(m, S[done + tid]);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, kWave));
+    if (((int)threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out_max, m);
 }
 
-__global__ void k_pack_traj(const pfh_cell_state* cell, const uint8_t* clsv,
-                            pfh_traj_packed_cell* out, int N, int L) {
-    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (n >= N) return;
-    const pfh_cell_state cs = cell[n];
-    const double V = cs.V > 0.0 ? cs.V : 1.0;
-    const double cx = (double)cs.gx0 + cs.Cx / V;
-    const double cy = (double)cs.gy0 + cs.Cy / V;
-    pfh_traj_packed_cell t;
-    t.global_id = cs.global_id;
-    t.cls = (int)clsv[n];
-    t.cx = (float)(cx - floor(cx / (double)L) * (double)L);
-    t.cy = (float)(cy - floor(cy / (double)L) * (double)L);
-    t.vx = cs.vx;
-    t.vy = cs.vy;
-    t.theta = cs.theta;
-    t.volume = (float)cs.V;
-    t.perim = (float)cs.perim;
-    t.gamma = cs.gamma;
-    t.v_A = cs.v_A;
-    t.phi_max = cs.phi_max;
-    out[n] = t;
-}
+// Trajectory records (k_pack_traj's fields, src/kernels.cu:1242-1265) of 64
+// cells per one-wave workgroup: the 64 CellStates (12 KiB) arrive in LDS by
+// 16 B loads, lane c packs cell c into LDS, and the 3 KiB of records leave by
+// 16 B stores (write-combined when out is mapped host memory).
+constexpr int kCellVecs = (int)(sizeof(pfh_cell_state) / 16);
+constexpr int kTrajVecs = (int)(sizeof(pfh_traj_packed_cell) / 16);
+static_assert(sizeof(pfh_cell_state) % 16 == 0 && sizeof(pfh_traj_packed_cell) % 16 == 0,
+              "16 B record vectors");
 
-__device__ __forceinline__ uint32_t part1by1(uint32_t x) {
-    x &= 0x0000FFFFu;
-    x = (x | (x << 8)) & 0x00FF00FFu;
-    x = (x | (x << 4)) & 0x0F0F0F0Fu;
-    x = (x | (x << 2)) & 0x33333333u;
-    x = (x | (x << 1)) & 0x55555555u;
-    return x;
-}
-
-// One-workgroup bitonic sort of (Morton key of the centroid, index).
-__global__ __launch_bounds__(1024) void k_morton_sort(const pfh_cell_state* cell,
-                                                      uint32_t* perm, int N,
-                                                      int M, int L) {
-    extern __shared__ unsigned long long ks[];
-    for (int i = (int)threadIdx.x; i < M; i += (int)blockDim.x) {
-        unsigned long long k = 0xFFFFFFFFFFFFFFFFull;
-        if (i < N) {
-            const pfh_cell_state cs = cell[i];
-            const double V = cs.V > 0.0 ? cs.V : 1.0;
-            const int cx = wrapi(cs.gx0 + (int)(cs.Cx / V), L);
-            const int cy = wrapi(cs.gy0 + (int)(cs.Cy / V), L);
-            k = ((unsigned long long)(part1by1((uint32_t)cx) |
-                                      (part1by1((uint32_t)cy) << 1)) << 32) |
-                (unsigned long long)(uint32_t)i;
-        }
-        ks[i] = k;
+__global__ __launch_bounds__(kWave) void k_pack_traj(const pfh_cell_state* cell,
+                                                     const uint8_t* clsv,
+                                                     pfh_traj_packed_cell* out, int N,
+                                                     int L) {
+    __shared__ u32x4 in[kWave * kCellVecs];
+    __shared__ u32x4 rec[kWave * kTrajVecs];
+    const int lane = (int)threadIdx.x;
+    const int base = (int)blockIdx.x * kWave;
+    const int cnt = min(kWave, N - base);
+    const u32x4* src = reinterpret_cast<const u32x4*>(cell + base);
+    for (int i = lane; i < cnt * kCellVecs; i += kWave) in[i] = src[i];
+    __syncthreads();
+    if (lane < cnt) {
+        const pfh_cell_state& cs = *reinterpret_cast<const pfh_cell_state*>(&in[lane * kCellVecs]);
+        pfh_traj_packed_cell& t = *reinterpret_cast<pfh_traj_packed_cell*>(&rec[lane * kTrajVecs]);
+        const double V = cs.V > 0.0 ? cs.V : 1.0;
+        const double Ld = (double)L;
+        const double cx = (double)cs.gx0 + cs.Cx / V;
+        const double cy = (double)cs.gy0 + cs.Cy / V;
+        t.global_id = cs.global_id;
+        t.cls = (int)clsv[base + lane];
+        t.cx = (float)(cx - floor(cx / Ld) * Ld);
+        t.cy = (float)(cy - floor(cy / Ld) * Ld);
+        t.vx = cs.vx;
+        t.vy = cs.vy;
+        t.theta = cs.theta;
+        t.volume = (float)cs.V;
+        t.perim = (float)cs.perim;
+        t.gamma = cs.gamma;
+        t.v_A = cs.v_A;
+        t.phi_max = cs.phi_max;
     }
     __syncthreads();
-    for (int k = 2; k <= M; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = (int)threadIdx.x; i < M; i += (int)blockDim.x) {
-                const int q = i ^ j;
-                if (q > i) {
-                    const bool up = ((i & k) == 0);
-                    if ((ks[i] > ks[q]) == up) {
-                        const unsigned long long tmp = ks[i];
-                        ks[i] = ks[q];
-                        ks[q] = tmp;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    u32x4* dst = reinterpret_cast<u32x4*>(out + base);
+    for (int i = lane; i < cnt * kTrajVecs; i += kWave) dst[i] = rec[i];
+}
+
+// ---- Morton ordering of the cell queue ----
+// Key of cell i: 2D Morton code (x in the even bits, y in the odd bits, 16
+// bits each) of its truncated centroid, wrapped into the domain
+// (src/kernels.cu:1271-1279). The queue is the cells sorted by (key, i).
+//
+// One workgroup of 1024 threads: a stable least-significant-digit radix sort
+// of the indices on 4-bit digits, entirely in LDS. Per pass, each wave ranks
+// its 64 elements in their digit bucket with 16 ballots, the per-(digit,
+// round, wave) counts are scanned in that order (stable across waves and
+// rounds of 1024 elements), and the indices move to their bucket positions.
+// Only as many passes run as the domain's coordinates have bits.
+constexpr int kMortonMax = 16384;
+constexpr int kMortonRounds = kMortonMax / kNT;
+constexpr int kDigitBits = 4;
+constexpr int kBuckets = 1 << kDigitBits;
+constexpr int kCountSlots = kBuckets * kMortonRounds * kWaves;   // 4096
+
+__device__ __forceinline__ uint32_t spread_bits16(uint32_t v) {
+    // bit i of v -> bit 2i of the result (v < 2^16)
+    uint32_t r = 0u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) r |= ((v >> i) & 1u) << (2 * i);
+    return r;
+}
+
+__global__ __launch_bounds__(kNT) void k_morton_sort(const pfh_cell_state* cell,
+                                                     uint32_t* perm, int N, int L,
+                                                     int passes) {
+    __shared__ uint32_t key[kMortonMax];
+    __shared__ uint16_t idx[2][kMortonMax];
+    __shared__ uint32_t cnt[kCountSlots];
+    __shared__ uint32_t wsum[kWaves];
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
+    const int rounds = (N + kNT - 1) / kNT;
+    for (int i = tid; i < N; i += kNT) {
+        const pfh_cell_state& cs = cell[i];
+        const double V = cs.V > 0.0 ? cs.V : 1.0;
+        const uint32_t x = (uint32_t)wrapi(cs.gx0 + (int)(cs.Cx / V), L);
+        const uint32_t y = (uint32_t)wrapi(cs.gy0 + (int)(cs.Cy / V), L);
+        key[i] = spread_bits16(x & 0xFFFFu) | (spread_bits16(y & 0xFFFFu) << 1);
+        idx[0][i] = (uint16_t)i;
     }
-    for (int i = (int)threadIdx.x; i < N; i += (int)blockDim.x)
-        perm[i] = (uint32_t)(ks[i] & 0xFFFFFFFFull);
+    __syncthreads();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    int cur = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * kDigitBits;
+        uint32_t digit[kMortonRounds], rank[kMortonRounds];
+        // 1. per-wave digit counts and ranks within the wave
+#pragma unroll
+        for (int r = 0; r < kMortonRounds; ++r) {
+            if (r >= rounds) break;
+            const int i = r * kNT + tid;
+            const uint32_t d = i < N ? (key[idx[cur][i]] >> shift) & (kBuckets - 1)
+                                     : (uint32_t)kBuckets;
+            uint64_t mine = 0ull;
+            uint32_t c = 0u;
+#pragma unroll
+            for (int k = 0; k < kBuckets; ++k) {
+                const uint64_t m = __ballot(d == (uint32_t)k);
+                if (d == (uint32_t)k) mine = m;
+                if (lane == k) c = (uint32_t)__popcll(m);
+            }
+            digit[r] = d;
+            rank[r] = (uint32_t)__popcll(mine & lt_mask);
+            if (lane < kBuckets) cnt[(lane * rounds + r) * kWaves + wave] = c;
+        }
+        __syncthreads();
+        // 2. exclusive scan of the counts in (digit, round, wave) order
+        const int slots = kBuckets * rounds * kWaves;
+        const int per = (slots + kNT - 1) / kNT;
+        uint32_t local = 0u;
+        for (int k = 0; k < per; ++k) {
+            const int s = tid * per + k;
+            if (s < slots) local += cnt[s];
+        }
+        uint32_t incl = local;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)incl, d, kWave);
+            if (lane >= d) incl += o;
+        }
+        if (lane == kWave - 1) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t run = incl - local;
+        for (int w = 0; w < wave; ++w) run += wsum[w];
+        for (int k = 0; k < per; ++k) {
+            const int s = tid * per + k;
+            if (s < slots) {
+                const uint32_t c = cnt[s];
+                cnt[s] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        // 3. stable moves into the other index buffer
+#pragma unroll
+        for (int r = 0; r < kMortonRounds; ++r) {
+            if (r >= rounds) break;
+            const int i = r * kNT + tid;
+            if (i < N)
+                idx[cur ^ 1][cnt[(digit[r] * rounds + r) * kWaves + wave] + rank[r]] =
+                    idx[cur][i];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int i = tid; i < N; i += kNT) perm[i] = idx[cur][i];
 }
 
 }  // namespace pfh
@@ -1413,11 +1577,10 @@ int pfh2d_configure_step(void) {
 }
 
 int pfh2d_configure_morton(int smem_bytes) {
+    // The sort keeps its keys and index buffers in static LDS (sized for
+    // 16384 cells); there is nothing to opt into on gfx950.
     if (smem_bytes <= 0 || smem_bytes > 160 * 1024) return invalid("morton smem");
-    return hip_status(hipFuncSetAttribute(
-                          reinterpret_cast<const void*>(pfh::k_morton_sort),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, smem_bytes),
-                      "hipFuncSetAttribute(k_morton_sort)");
+    return PFH_OK;
 }
 
 int pfh2d_step_grid(int device) {
@@ -1511,8 +1674,10 @@ int pfh2d_init_tiles(float* phi_a, float* phi_b, pfh_cell_state* cell,
                                       !seed_cx || !seed_cy)))
         return invalid("init_tiles");
     if (N == 0) return PFH_OK;
-    hipLaunchKernelGGL(pfh::k_init_tiles, dim3(N), dim3(256), 0, as_stream(stream),
-                       phi_a, phi_b, cell, cls, N, L, seed_cx, seed_cy,
+    if ((uintptr_t)phi_a % 16 || (uintptr_t)phi_b % 16)
+        return invalid("init_tiles: tiles must be 16 B aligned");
+    hipLaunchKernelGGL(pfh::k_init_tiles, dim3((unsigned)N, pfh::kInitBands), dim3(256), 0,
+                       as_stream(stream), phi_a, phi_b, cell, cls, L, seed_cx, seed_cy,
                        radius_eff, kappa_iface);
     return launch_status("k_init_tiles");
 }
@@ -1533,19 +1698,19 @@ int pfh2d_scatter_all(const float* phi, const pfh_cell_state* cell,
         (N > 0 && (!phi || !cell || !cls || !S || !flags)))
         return invalid("scatter_all");
     if (N == 0) return PFH_OK;
-    hipLaunchKernelGGL(pfh::k_scatter_all, dim3(N), dim3(256), 0, as_stream(stream),
-                       phi, cell, cls, S, N, L, P, flags);
+    hipLaunchKernelGGL(pfh::k_scatter_all, dim3((unsigned)N, pfh::kScatterBands), dim3(256), 0,
+                       as_stream(stream), phi, cell, cls, S, L, P, flags);
     return launch_status("k_scatter_all");
 }
 
 int pfh2d_zero_u32(uint32_t* p, size_t n, void* stream) {
     if (n == 0) return PFH_OK;
     if (!p) return invalid("zero_u32");
-    size_t blocks = (n + 255) / 256;
+    size_t blocks = (n / 4 + 255) / 256 + 1;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(pfh::k_zero_u32, dim3((unsigned)blocks), dim3(256), 0,
-                       as_stream(stream), p, n);
-    return launch_status("k_zero_u32");
+    hipLaunchKernelGGL(pfh::k_fill_u32, dim3((unsigned)blocks), dim3(256), 0,
+                       as_stream(stream), p, n, 0u);
+    return launch_status("k_fill_u32");
 }
 
 int pfh2d_verify_cells(const float* phi, const pfh_cell_state* cell,
@@ -1554,16 +1719,19 @@ int pfh2d_verify_cells(const float* phi, const pfh_cell_state* cell,
     if (N < 0 || (N > 0 && (!phi || !cell || !cls || !out_V || !out_outside_max)))
         return invalid("verify_cells");
     if (N == 0) return PFH_OK;
-    hipLaunchKernelGGL(pfh::k_verify_cells, dim3(N), dim3(1024), 0,
-                       as_stream(stream), phi, cell, cls, N, out_V, out_outside_max);
+    if ((uintptr_t)phi % 16) return invalid("verify_cells: tiles must be 16 B aligned");
+    hipLaunchKernelGGL(pfh::k_verify_cells, dim3((unsigned)N), dim3(pfh::kNT), 0,
+                       as_stream(stream), phi, cls, out_V, out_outside_max);
     return launch_status("k_verify_cells");
 }
 
 int pfh2d_verify_S(const uint32_t* S, size_t n, uint32_t* out_max, void* stream) {
     if (!out_max || (n > 0 && !S)) return invalid("verify_S");
     if (n == 0) return PFH_OK;
-    hipLaunchKernelGGL(pfh::k_verify_S, dim3(256), dim3(256), 0, as_stream(stream),
-                       S, n, out_max);
+    size_t blocks = (n / 4 + 255) / 256 + 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(pfh::k_verify_S, dim3((unsigned)blocks), dim3(256), 0,
+                       as_stream(stream), S, n, out_max);
     return launch_status("k_verify_S");
 }
 
@@ -1584,22 +1752,25 @@ int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,
                     pfh_traj_packed_cell* out, int N, int L, void* stream) {
     if (N < 0 || L <= 0 || (N > 0 && (!cell || !cls || !out))) return invalid("pack_traj");
     if (N == 0) return PFH_OK;
-    hipLaunchKernelGGL(pfh::k_pack_traj, dim3((N + 127) / 128), dim3(128), 0,
-                       as_stream(stream), cell, cls, out, N, L);
+    if ((uintptr_t)out % 16 || (uintptr_t)cell % 16)
+        return invalid("pack_traj: cell and out must be 16 B aligned");
+    hipLaunchKernelGGL(pfh::k_pack_traj, dim3((unsigned)((N + pfh::kWave - 1) / pfh::kWave)),
+                       dim3(pfh::kWave), 0, as_stream(stream), cell, cls, out, N, L);
     return launch_status("k_pack_traj");
 }
 
 int pfh2d_morton_sort(const pfh_cell_state* cell, uint32_t* perm, int N, int M,
                       int L, void* stream) {
-    if (N <= 0 || M < N || (M & (M - 1)) != 0 || M > 16384 || L <= 0 || !cell || !perm)
+    if (N <= 0 || M < N || (M & (M - 1)) != 0 || M > pfh::kMortonMax || L <= 0 || !cell ||
+        !perm)
         return invalid("morton_sort");
-    const size_t smem = (size_t)M * sizeof(unsigned long long);
-    if (smem > 48 * 1024) {
-        const int st = pfh2d_configure_morton((int)smem);
-        if (st != PFH_OK) return st;
-    }
-    hipLaunchKernelGGL(pfh::k_morton_sort, dim3(1), dim3(1024), smem,
-                       as_stream(stream), cell, perm, N, M, L);
+    // Digit passes needed for the key bits the domain can produce: both
+    // coordinates are below min(L, 2^16).
+    int coord_bits = 0;
+    while (coord_bits < 16 && ((L - 1) >> coord_bits) != 0) ++coord_bits;
+    const int passes = (2 * coord_bits + pfh::kDigitBits - 1) / pfh::kDigitBits;
+    hipLaunchKernelGGL(pfh::k_morton_sort, dim3(1), dim3(pfh::kNT), 0, as_stream(stream),
+                       cell, perm, N, L, passes);
     return launch_status("k_morton_sort");
 }
 

@@ -933,27 +933,53 @@ __global__ __launch_bounds__(256) void k3_clear(uint32_t* p, size_t n4) {
         d[i] = z4;
 }
 
-// k_pack_trajectory, src/pf3d/kernels.cu:2323-2356 (periodic geometry)
-__global__ void k3_pack_traj(const pfh3d_cell_state* cells, pfh3d_traj_cell* out, int N) {
-    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (n >= N) return;
-    const pfh3d_cell_state s = cells[n];
-    const double inv = s.V > 0.0 ? 1.0 / s.V : 0.0;
-    pfh3d_traj_cell r;
-    r.global_id = s.global_id;
-    r.x = (double)s.origin_x + s.Cx * inv;
-    r.y = (double)s.origin_y + s.Cy * inv;
-    r.z = (double)s.origin_z + s.Cz * inv;
-    r.volume = s.V;
-    r.surface = s.surface;
-    r.wall_overlap = s.wall_overlap;
-    r.outside_slit_volume = s.outside_slit_volume;
-    r.velocity_x = s.velocity_x; r.velocity_y = s.velocity_y; r.velocity_z = s.velocity_z;
-    r.polarity_x = s.polarity_x; r.polarity_y = s.polarity_y; r.polarity_z = s.polarity_z;
-    r.gamma = s.gamma;
-    r.v_A = s.v_A;
-    r.phi_max = s.phi_max;
-    out[n] = r;
+// Trajectory records (k_pack_trajectory's fields, src/pf3d/kernels.cu:2323-2356,
+// periodic geometry: no height offset) of 64 cells per one-wave workgroup:
+// the 64 CellState3D records (16 KiB) arrive in LDS by 16 B loads, lane c
+// packs cell c's record into LDS, and the 6.5 KiB of records leave by 8 B
+// stores (a record is 104 B).
+constexpr int kCell3Vecs = (int)(sizeof(pfh3d_cell_state) / 16);
+constexpr int kTraj3Words = (int)(sizeof(pfh3d_traj_cell) / 8);
+static_assert(sizeof(pfh3d_cell_state) % 16 == 0 && sizeof(pfh3d_traj_cell) % 8 == 0,
+              "record vectors");
+
+__global__ __launch_bounds__(64) void k3_pack_traj(const pfh3d_cell_state* cells,
+                                                   pfh3d_traj_cell* out, int N) {
+    __shared__ u32x4 in[64 * kCell3Vecs];
+    __shared__ uint64_t rec[64 * kTraj3Words];
+    const int lane = (int)threadIdx.x;
+    const int base = (int)blockIdx.x * 64;
+    const int cnt = min(64, N - base);
+    const u32x4* src = reinterpret_cast<const u32x4*>(cells + base);
+    for (int i = lane; i < cnt * kCell3Vecs; i += 64) in[i] = src[i];
+    __syncthreads();
+    if (lane < cnt) {
+        const pfh3d_cell_state& s =
+            *reinterpret_cast<const pfh3d_cell_state*>(&in[lane * kCell3Vecs]);
+        pfh3d_traj_cell& r = *reinterpret_cast<pfh3d_traj_cell*>(&rec[lane * kTraj3Words]);
+        const double inv = s.V > 0.0 ? 1.0 / s.V : 0.0;
+        r.global_id = s.global_id;
+        r.x = (double)s.origin_x + s.Cx * inv;
+        r.y = (double)s.origin_y + s.Cy * inv;
+        r.z = (double)s.origin_z + s.Cz * inv;
+        r.volume = s.V;
+        r.surface = s.surface;
+        r.wall_overlap = s.wall_overlap;
+        r.outside_slit_volume = s.outside_slit_volume;
+        r.velocity_x = s.velocity_x;
+        r.velocity_y = s.velocity_y;
+        r.velocity_z = s.velocity_z;
+        r.polarity_x = s.polarity_x;
+        r.polarity_y = s.polarity_y;
+        r.polarity_z = s.polarity_z;
+        r.gamma = s.gamma;
+        r.v_A = s.v_A;
+        r.phi_max = s.phi_max;
+        reinterpret_cast<uint32_t*>(&r)[25] = 0u;      // the 4 B of tail padding
+    }
+    __syncthreads();
+    uint64_t* dst = reinterpret_cast<uint64_t*>(out + base);
+    for (int i = lane; i < cnt * kTraj3Words; i += 64) dst[i] = rec[i];
 }
 
 }  // namespace pfh3
@@ -1101,7 +1127,9 @@ int pfh3d_step(const pfh3d_step_args* a, int slabs, void* stream) {
 int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out, int N,
                           void* stream) {
     if (!cells || !out || N <= 0) return invalid("pack_trajectory");
-    hipLaunchKernelGGL(pfh3::k3_pack_traj, dim3((unsigned)((N + 127) / 128)), dim3(128), 0,
+    if ((uintptr_t)cells % 16 || (uintptr_t)out % 8)
+        return invalid("pack_trajectory: cells 16 B and out 8 B aligned");
+    hipLaunchKernelGGL(pfh3::k3_pack_traj, dim3((unsigned)((N + 63) / 64)), dim3(64), 0,
                        as_stream(stream), cells, out, N);
     return launch_status("k3_pack_traj");
 }

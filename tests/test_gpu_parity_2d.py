@@ -179,6 +179,27 @@ def test_c1_two_workgroups_bit_exact(sim_mod, oracle):
         assert_state_equal(sim, orc, f"C1 two workgroups, step {sim.steps_done}")
 
 
+def test_interacting_cluster_bit_exact(sim_mod, oracle):
+    """Four overlapping R=20 cells (the oracle's translation, transpose and
+    dense-evaluator pins run on this state) with tumbles on most steps:
+    10 steps bit-exact, with nonzero interaction integrals."""
+    cx = np.array([143.0, 177.25, 144.5, 178.0], np.float32)
+    cy = np.array([143.5, 144.0, 177.0, 178.75], np.float32)
+    p = params.SimParams(Nx=320, Ny=320, num_cells=4, rho=0.05, target_radius=20.0,
+                         tau=0.04, seed=11, polarity_seed=12)
+    wl = params.Workload2D("cluster", p, cx, cy)
+    say("interacting 4-cell cluster L=320, 10 steps vs oracle")
+    sim = sim_mod.Sim2D(wl)
+    orc = oracle.OracleSim2D(wl, threads=4)
+    for k in range(2):
+        sim.step(5)
+        orc.step(5)
+        sim.sync()
+        assert_state_equal(sim, orc, f"cluster step {sim.steps_done}")
+    c = sim.cells()
+    assert np.abs(c["Ix"]).max() > 1e-3 and c["tumble_ctr"].sum() > 0
+
+
 def test_fatal_flags_match_oracle(sim_mod, oracle):
     wl, phi, S, cells, cls = _forced_state(oracle)
     cells = cells.copy()
