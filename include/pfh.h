@@ -252,6 +252,27 @@ int pfh2d_pack_traj(const pfh_cell_state* cell, const uint8_t* cls,
 int pfh2d_morton_sort(const pfh_cell_state* cell, uint32_t* perm, int N,
                       int M, int L, void* stream);
 
+/* ---- 2D host-side set-up (no GPU work) ---- */
+/* Philox4x32-10 (common/philox.cuh:32-59): out = the ten-round block of
+ * counter ctr under key. The kernels' own implementation, for host callers. */
+int pfh_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* philox_uniform53 (common/philox.cuh:61-67): ((a << 32 | b) >> 11) 2^-53. */
+double pfh_uniform53(uint32_t a, uint32_t b);
+/* domain_side_for (include/params.cuh:562-569): ceil(sqrt(N pi R^2 / rho)). */
+int pfh2d_domain_side(int num_cells, double radius, double rho);
+/* s_pitch_for (include/params.cuh:573-580): 32 ceil(side / 32). */
+int pfh2d_s_pitch(int side);
+/* ic_theta (include/kernels.cuh:153-158): initial polarity angle of a cell. */
+float pfh2d_ic_theta(int32_t gid, uint64_t polarity_stream);
+/* ic_v_A (include/kernels.cuh:161-172): lognormal per-cell speed, median v_A. */
+double pfh2d_ic_v_A(int32_t gid, uint64_t seed, double v_A, double sigma);
+/* The default grid + jitter placement of Sim::seed_positions
+ * (src/sim.cu:108-129): binary32 centres of N cells in an L x L domain. */
+int pfh2d_grid_jitter_centres(int N, int L, uint64_t seed, float* cx, float* cy);
+/* palmieri_centre_table_fnv1a64 (include/palmieri_initializer.hpp:105-122):
+ * FNV-1a over the little-endian words x0, y0, x1, y1, ... */
+uint64_t pfh_centre_table_fnv1a64(const float* x, const float* y, int n);
+
 /* ====================================================================== */
 /* ---- 3D: include/pf3d/kernels.cuh (periodic XYZ, cubic bricks) ---- */
 

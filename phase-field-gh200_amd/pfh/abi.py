@@ -164,6 +164,14 @@ SIGNATURES = {
     "pfh2d_verify_S": (_I, [_P, _SZ, _P, _P]),
     "pfh2d_pack_traj": (_I, [_P, _P, _P, _I, _I, _P]),
     "pfh2d_morton_sort": (_I, [_P, _P, _I, _I, _I, _P]),
+    "pfh_philox4x32_10": (_I, [_P, _P, _P]),
+    "pfh_uniform53": (C.c_double, [C.c_uint32, C.c_uint32]),
+    "pfh2d_domain_side": (_I, [_I, C.c_double, C.c_double]),
+    "pfh2d_s_pitch": (_I, [_I]),
+    "pfh2d_ic_theta": (_F, [C.c_int32, C.c_uint64]),
+    "pfh2d_ic_v_A": (C.c_double, [C.c_int32, C.c_uint64, C.c_double, C.c_double]),
+    "pfh2d_grid_jitter_centres": (_I, [_I, _I, C.c_uint64, _P, _P]),
+    "pfh_centre_table_fnv1a64": (C.c_uint64, [_P, _P, _I]),
     "pfh3d_init_spheres": (_I, [_P, _P, _P, _P, _I, _I, _I, _F, _F, _P]),
     "pfh3d_clear_S": (_I, [_P, _I, _I, _P]),
     "pfh3d_scatter_current": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),

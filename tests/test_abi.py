@@ -72,3 +72,43 @@ def test_launch_rejects_small_domain():
     # L <= 286: a window would alias onto itself (include/params.cuh:666-680)
     assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 0) == -1
     assert b"286" in lib.pfh_last_error()
+
+
+def _fnv1a64(data: bytes) -> int:
+    h = 0xCBF29CE484222325
+    for b in data:
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_host_setup_draws_match_the_restated_placement():
+    """The library's host-side draws (Philox, u53, ic_theta, ic_v_A, the grid
+    + jitter placement) equal the package's own restatement, which the
+    reference's Philox known-answer vectors pin; FNV-1a follows its
+    published definition (the 'a' vector) over x0, y0, x1, y1 little-endian."""
+    lib = abi.load()
+    from pfh import params
+    out = (C.c_uint32 * 4)()
+    for ctr, key in [((0, 0, 0, 0), (0, 0)), ((1, 2, 3, 4), (5, 6)),
+                     ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2)]:
+        assert lib.pfh_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out) == 0
+        assert tuple(out) == tuple(params.philox4x32_10(*ctr, *key))
+    assert lib.pfh_uniform53(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 2.0 ** -53
+    assert lib.pfh_uniform53(0, 0x800) == 2.0 ** -53
+    for gid in (0, 1, 7, 1023):
+        assert np.float32(lib.pfh2d_ic_theta(gid, 7)) == params.ic_theta(gid, 7)
+        assert lib.pfh2d_ic_v_A(gid, 1234, 0.01, 0.3) == params.ic_v_A(gid, 1234, 0.01, 0.3)
+    assert lib.pfh2d_ic_v_A(3, 1234, 0.01, 0.0) == 0.01
+    for N, L, seed in [(16, 384, 1), (4, 301, 20260823), (1000, 2048, 1234)]:
+        cx = np.zeros(N, np.float32)
+        cy = np.zeros(N, np.float32)
+        assert lib.pfh2d_grid_jitter_centres(N, L, seed, cx.ctypes.data, cy.ctypes.data) == 0
+        ex, ey = params.grid_jitter_centres(N, L, seed)
+        assert cx.tobytes() == ex.tobytes() and cy.tobytes() == ey.tobytes()
+        xy = np.stack([cx, cy], axis=1).astype("<f4").tobytes()
+        assert lib.pfh_centre_table_fnv1a64(cx.ctypes.data, cy.ctypes.data, N) == _fnv1a64(xy)
+    assert _fnv1a64(b"a") == 0xAF63DC4C8601EC8C
+    assert lib.pfh2d_domain_side(16, 20.0, 0.13671) == 384
+    assert lib.pfh2d_domain_side(4, 12.0, 0.02) == 301
+    assert lib.pfh2d_s_pitch(301) == 320 and lib.pfh2d_s_pitch(2048) == 2048
+    assert lib.pfh2d_domain_side(0, 20.0, 0.5) < 0

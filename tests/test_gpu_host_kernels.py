@@ -45,7 +45,8 @@ def _mixed_state(oracle, L=701, N=12, seed=3):
     cx, cy = params.harness_centres(N, L, 4, 6.0, seed)
     # cells 0 and 11 wrap across the periodic boundary
     cx[0], cy[0] = np.float32(L - 3.25), np.float32(2.5)
-    cx[11], cy[11] = np.float32(1.75), np.float32(L - 10.0)
+    if N > 11:
+        cx[11], cy[11] = np.float32(1.75), np.float32(L - 10.0)
     p = params.SimParams(Nx=L, Ny=L, num_cells=N, rho=0.05, target_radius=20.0, seed=seed)
     cells = oracle._host_cells(p, N)
     return p, cls, cells, cx, cy
