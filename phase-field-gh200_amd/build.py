@@ -1,8 +1,9 @@
-"""Build recipe of the gfx950 library (lib/libpfh.so) and the pf2d CLI.
+"""Build recipe of the gfx950 library (lib/libpfh.so) and the pf2d driver.
 
-Compiles every HIP source under csrc/ for gfx950 and links it with the C++
-host driver into one shared library exporting the C ABI of include/pfh.h.
-Runs on a machine without a GPU (hipcc cross-compiles).
+Compiles every HIP source under csrc/ for gfx950 into one shared library
+exporting the C ABI of include/pfh.h, then the C++ host driver under host/
+(the pf2d command-line simulator) into lib/pf2d, linked against that
+library. Runs on a machine without a GPU (hipcc cross-compiles).
 """
 from __future__ import annotations
 
@@ -54,7 +55,8 @@ def _stale(out: Path, deps: list[Path]) -> bool:
 
 
 def build(force: bool = True) -> Path:
-    """Compile csrc/ into lib/libpfh.so (and lib/pf2d). Returns the .so path."""
+    """Compile csrc/ into lib/libpfh.so and host/ into lib/pf2d. Returns the
+    .so path."""
     LIB.mkdir(exist_ok=True)
     hip_srcs = sorted(CSRC.glob("*.hip"))
     cpp_srcs = sorted(CSRC.glob("*.cpp"))
@@ -72,11 +74,15 @@ def build(force: bool = True) -> Path:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
           *map(str, objs)])
     os.replace(tmp, so)
-    main_src = HERE / "tools" / "pf2d_main.cpp"
-    if main_src.exists():
-        exe = LIB / "pf2d"
-        _run([HIPCC, *HIP_FLAGS, "-x", "hip", str(main_src), "-o", str(exe),
-              "-L", str(LIB), "-lpfh", f"-Wl,-rpath,{LIB}"])
+    # The host driver: host-only C++ that reaches the GPU through the C ABI.
+    host_srcs = sorted((HERE / "host").glob("*.cpp"))
+    host_hdrs = sorted((HERE / "host").glob("*.hpp")) + [REPO / "include" / "pfh.h"]
+    exe = LIB / "pf2d"
+    if host_srcs and (force or _stale(exe, host_srcs + host_hdrs + [so])):
+        tmp_exe = LIB / "pf2d.tmp"
+        _run([HIPCC, "-O2", "-std=c++17", "-I", str(REPO / "include"), *map(str, host_srcs),
+              "-o", str(tmp_exe), "-L", str(LIB), "-lpfh", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp_exe, exe)
     return so
 
 
