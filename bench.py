@@ -72,11 +72,18 @@ def read_traffic(path: Path = TRAFFIC_SUMMARY, key: str = "traffic_bytes_per_lau
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
-    """The CPU oracle (binary32, the kernel's operation order) on the same
-    workload, timed on this host's cores after two warm-up steps."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(wl, threads: int, budget_s: float, max_steps: int) -> tuple[float, int, float]:
     from oracle import pfo
-    threads = min(16, os.cpu_count() or 1)
     orc = pfo.OracleSim2D(wl, threads=threads)
     orc.step(2)
     t0 = time.perf_counter()
@@ -85,12 +92,35 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
         orc.step(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 200:
+        if el >= budget_s or steps >= max_steps:
             break
-    return {"value": wl.params.num_cells * steps / el, "unit": "cell-steps/s",
-            "cores": threads, "kind": "port",
+    return wl.params.num_cells * steps / el, steps, el
+
+
+def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
+    """The CPU oracle (binary32, the kernel's operation order) on the same
+    workload, timed on this host's cores after two warm-up steps; beside it
+    the same on one thread and the 384^2 / 16-cell workload's 1000 steps."""
+    from pfh import params
+    threads = min(16, os.cpu_count() or 1)
+    rate, steps, el = _oracle_rate(wl, threads, budget_s, 200)
+    rate1, steps1, el1 = _oracle_rate(wl, 1, 0.4 * budget_s, 20)
+    c1 = params.workload_c1()
+    from oracle import pfo
+    orc = pfo.OracleSim2D(c1, threads=threads)
+    t0 = time.perf_counter()
+    orc.step(1000)
+    el_c1 = time.perf_counter() - t0
+    return {"value": rate, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{steps} steps x {wl.params.num_cells} cells of the same workload "
-                      f"after 2 warm-up steps ({el:.1f} s wall)"}
+                      f"after 2 warm-up steps ({el:.1f} s wall)",
+            "one_thread": {"value": rate1, "unit": "cell-steps/s",
+                           "sample": f"{steps1} steps x {wl.params.num_cells} cells, 1 thread "
+                                     f"({el1:.1f} s wall)"},
+            "c1_full": {"workload": c1.name, "steps": 1000, "wall_s": el_c1,
+                        "value": c1.params.num_cells * 1000 / el_c1, "unit": "cell-steps/s",
+                        "cores": threads}}
 
 
 def q_nonzero_3d(sim) -> int:
@@ -196,6 +226,66 @@ def bench_surrogate(blocks: int, path=None) -> dict | None:
     return r
 
 
+def q_nonzero_ensemble(ens) -> tuple[int, int]:
+    """Window pixels and pixels with q(phi) != 0 over every replica's cells."""
+    import torch
+    from pfh import abi
+    phi = ens.phi[ens.steps_done % 2].view(ens.N, abi.TILE_PITCH, abi.TILE_PITCH)
+    W = Q = 0
+    for c in set(int(v) for v in ens.cls.cpu().numpy()):
+        wx, wy, tx0, ty0 = abi.CLASSES[c]
+        idx = torch.nonzero(ens.cls == c).flatten()
+        W += int(idx.numel()) * wx * wy
+        for chunk in torch.split(idx, 1024):
+            win = phi[chunk, ty0:ty0 + wy, tx0:tx0 + wx]
+            q = torch.round(torch.clamp(win * win, max=4.0) * 134217728.0)
+            Q += int((q != 0).sum().item())
+    return W, Q
+
+
+def bench_batched(steps: int, warmup: int) -> dict:
+    """Throughput of the batched ensemble: 64 independent replicas (sides
+    256/512/1024, 16 to 750 cells at R=20, seeds 100..163) advanced by one
+    persistent launch per step, replayed from six-step HIP graphs. Bytes per
+    step: sum over replicas of N_r (12 W + 8 Q) + 4 P_r L_r."""
+    import torch
+    from pfh import ensemble, params
+    wls = params.ensemble_c4()
+    ens = ensemble.Ensemble2D(wls)
+    ens.capture_graph()
+    ens.run(warmup)
+    ens.sync()
+    W0, Q0 = q_nonzero_ensemble(ens)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(ens.stream)
+    ens.run(steps)
+    e1.record(ens.stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    W1, Q1 = q_nonzero_ensemble(ens)
+    if ens.fatal():
+        return {"error": "fatal integrity flags", "flags": ens.flag_counts().tolist()}
+    ms_per_step = ms / steps
+    bytes_per_step = 12.0 * 0.5 * (W0 + W1) + 8.0 * 0.5 * (Q0 + Q1) + 4.0 * ens.words
+    achieved = bytes_per_step / (ms_per_step * 1e-3) / 1e9
+    from collections import Counter
+    return {
+        "metric": "cell-steps/s, batched 2D ensemble of 64 replicas",
+        "value": ens.N * steps / (ms / 1e3), "unit": "cell-steps/s",
+        "steps": steps, "warmup": warmup, "ms_per_step": ms_per_step, "dtype": "f32",
+        "config": {"workload": "ensemble_c4_64", "replicas": ens.R, "cells": ens.N,
+                   "sides": dict(Counter(ens.Ls)), "radius": 20.0, "seeds": "100..163",
+                   "placement": "grid + jitter per replica", "graph_steps": 6},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_step": bytes_per_step,
+                     "window_pixels_per_step": 0.5 * (W0 + W1),
+                     "q_nonzero_per_step": 0.5 * (Q0 + Q1)},
+    }
+
+
 def calibrate_copy(mib: int) -> int:
     import ctypes as C
 
@@ -220,8 +310,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1200)
     ap.add_argument("--warmup", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["all", "2d", "3d"], default="all",
-                    help="2d: the headline 2048^2 step only; 3d: the 256^3 step only")
+    ap.add_argument("--workload", choices=["all", "2d", "3d", "batched"], default="all",
+                    help="2d: the headline 2048^2 step only; 3d: the 256^3 step only; "
+                         "batched: the 64-replica ensemble only")
+    ap.add_argument("--steps-batched", type=int, default=120)
     ap.add_argument("--steps3d", type=int, default=600)
     ap.add_argument("--warmup3d", type=int, default=40)
     ap.add_argument("--sur-blocks", type=int, default=25,
@@ -242,6 +334,9 @@ def main() -> int:
         print(json.dumps(bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)),
               flush=True)
         return 0
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args.steps_batched, 12)), flush=True)
+        return 0
 
     import torch
     from pfh import params, sim2d
@@ -249,8 +344,10 @@ def main() -> int:
     wl = params.workload_c2()
     sim = sim2d.Sim2D(wl, device="cuda:0")
     sim.capture_graph()
-    warm = ((args.warmup + 5) // 6) * 6
-    steps = max(6, (args.steps // 6) * 6)
+    # exactly the requested counts: whole six-step graph replays, single
+    # launches of the same kernel for the rest
+    warm = max(0, args.warmup)
+    steps = max(1, args.steps)
     sim.run(warm)
     sim.sync()
     q0 = q_nonzero_count(sim)
@@ -297,6 +394,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                                       "calibrated on a copy kernel, committed in "
+                                       "profiles/" + TRAFFIC_SUMMARY.name + " (not "
+                                       "measured in this run)",
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "window_pixels_per_launch": W, "q_nonzero_per_launch": Qbar},
         "surrogate": None,
@@ -315,6 +416,8 @@ def main() -> int:
         torch.cuda.empty_cache()
     if args.workload == "all":
         out["pf3d"] = bench_3d(args.steps3d, args.warmup3d, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(args.steps_batched, 12)
     print(json.dumps(out), flush=True)
     return 0
 

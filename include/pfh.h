@@ -129,6 +129,19 @@ typedef struct pfh2d_step_args {
     unsigned long long clear_ahead_words;
 } pfh2d_step_args;
 
+/* One replica of a batched 2D step (no reference counterpart; the batched
+ * ensemble generalises the persistent queue of src/kernels.cu:948-1042 to
+ * tickets over every replica's cells): its periodic side L, S row pitch P,
+ * the word offset of its aggregate inside each S slot arena, and its Philox
+ * streams (seed, and the polarity stream of its tumbles). 32 bytes. */
+typedef struct pfh2d_replica {
+    unsigned long long S_offset;
+    int32_t L;
+    int32_t P;
+    unsigned long long seed;
+    unsigned long long polarity_seed;
+} pfh2d_replica;
+
 /* Kernel resource report: include/kernels.cuh:230-244 (KernelStats). */
 typedef struct pfh_kernel_stats {
     int regs;             /* architected VGPRs per thread */
@@ -149,6 +162,7 @@ _Static_assert(offsetof(pfh_cell_state, advection) == 128, "advection moved");
 _Static_assert(offsetof(pfh_cell_state, reserved) == 120, "reserved moved");
 _Static_assert(offsetof(pfh_cell_state, M_pf) == 188, "M_pf moved");
 _Static_assert(sizeof(pfh_traj_packed_cell) == 48, "traj record size");
+_Static_assert(sizeof(pfh2d_replica) == 32, "replica record size");
 #else
 static_assert(sizeof(pfh_cell_state) == 192, "pfh_cell_state must be 192 B");
 static_assert(alignof(pfh_cell_state) == 64, "pfh_cell_state alignment");
@@ -156,6 +170,7 @@ static_assert(offsetof(pfh_cell_state, advection) == 128, "advection moved");
 static_assert(offsetof(pfh_cell_state, reserved) == 120, "reserved moved");
 static_assert(offsetof(pfh_cell_state, M_pf) == 188, "M_pf moved");
 static_assert(sizeof(pfh_traj_packed_cell) == 48, "traj record size");
+static_assert(sizeof(pfh2d_replica) == 32, "replica record size");
 #endif
 
 /* ---- library ---- */
@@ -206,6 +221,23 @@ int pfh2d_launch_step_if(const pfh2d_step_args* args, const int* run_if, int gri
 int pfh2d_launch_check_step_if(const pfh2d_step_args* args, const int* run_if, int grid,
                                void* stream);
 
+/* Batched step: one persistent launch advances every cell of many
+ * independent replicas by one step (generalises launch_step,
+ * src/kernels.cu:1329-1363, and the ticket queue of :948-1042 to an
+ * ensemble). `a` describes the ensemble as one population: phi_in/phi_out
+ * hold every replica's tiles, cell/cell_cls/perm every replica's cells (perm
+ * is the queue order over all N = sum N_r cells; each cell's global_id is
+ * its id within its replica), S_rd/S_sc/S_cl are the bases of the three S
+ * slot arenas and clear_ahead_words is the arena size; L, P, seed and
+ * polarity_seed of `a` are not read. replicas (device, num_replicas records)
+ * and cell_replica (device, N entries: the replica of each cell) give each
+ * cell's domain. max_side / max_pitch bound every replica's L and P; sides
+ * from 256 are accepted: a replica narrower than 287 pixels cannot use the
+ * 286-wide fallback class (a cell that would need it raises
+ * CLASS_EXHAUSTED). Capture-safe; allocates nothing. */
+int pfh2d_launch_batch_step(const pfh2d_step_args* a, const pfh2d_replica* replicas,
+                            const uint16_t* cell_replica, int num_replicas, int max_side,
+                            int max_pitch, int grid, void* stream);
 /* Replaces query_kernel_stats (include/kernels.cuh:243-244) for k_step. */
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out);
 
@@ -383,6 +415,56 @@ static_assert(sizeof(pfh3d_moment_partial) == 96, "MomentPartial3D is 96 B");
 static_assert(sizeof(pfh3d_traj_cell) == 104, "TrajPackedCell3D is 104 B");
 static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
 #endif
+
+/* Parameter record of a PF3D schema-1 checkpoint: ParamsRecord3D
+ * (common/checkpoint_format_3d.h:176-228), packed, 288 B, every offset as the
+ * reference pins it. This build writes and reads the periodic-XYZ cube
+ * (boundary_flags 7, Nx = Ny = Nz) with base bricks only. */
+#pragma pack(push, 1)
+typedef struct pfh3d_ckpt_params {
+    int64_t Nx, Ny, Nz;
+    double dx, dy, dz;
+    uint32_t brick_edge;
+    uint32_t boundary_flags;
+    double dt, t_end, lambda, gamma_normal, gamma_soft, soft_fraction, kappa;
+    double target_radius, mu, v_A, v_A_sigma, xi, tau, volume_fraction;
+    uint64_t seed, polarity_seed;
+    int64_t print_interval, full_moment_every, verify_every;
+    uint32_t integrator_id;        /* 1: explicit Euler */
+    uint32_t stencil_id;           /* 2: 27-point */
+    uint32_t rng_id;               /* 1: Philox4x32-10 */
+    uint32_t trajectory_interval;
+    uint64_t initialization_hash;
+    double aging_time;
+    uint64_t promoted_measure_policy;
+    uint64_t promoted_measure_auto_wave_ctas;
+    int64_t channel_height, channel_padding;
+    double wall_kappa, wall_width;
+} pfh3d_ckpt_params;
+#pragma pack(pop)
+
+/* CRC-64/ECMA-182 (common/checkpoint_format_3d.h:78-115): polynomial
+ * 0x42F0E1EBA9EA3693, MSB first, no reflection, init and final xor 0;
+ * continues `crc` over `bytes` bytes. Host only. */
+uint64_t pfh_crc64_ecma(uint64_t crc, const void* data, size_t bytes);
+/* Replaces checkpoint_write_3d (src/pf3d/checkpoint.cu:1009-1235): writes a
+ * PF3D schema-1 file (header, params, N x {record, B^3 cube}) through
+ * <path>.tmp and a rename, with the file CRC. cells: N host records; phi:
+ * the N device cubes of the current field. Host function; synchronises
+ * `stream` per cube. */
+int pfh3d_checkpoint_write(const char* path, int64_t step, double time,
+                           const pfh3d_ckpt_params* params, const pfh3d_cell_state* cells,
+                           int N, int B, const float* phi, uint64_t base_measure_shards,
+                           void* stream);
+/* Replaces checkpoint_probe_3d / checkpoint_load_3d
+ * (src/pf3d/checkpoint.cu:819-1007): validates header, parameters, every
+ * record and the file CRC; fills params, step, time, shards, N and B. With
+ * cells and phi NULL it only probes; otherwise cells (N host records) and phi
+ * (N device cubes) receive the state. Errors name the reason in
+ * pfh_last_error. */
+int pfh3d_checkpoint_read(const char* path, pfh3d_ckpt_params* params_out, int64_t* step,
+                          double* time, uint64_t* base_measure_shards, int* N, int* B,
+                          pfh3d_cell_state* cells, float* phi, void* stream);
 
 /* Replaces launch_initialize_spheres (kernels.cuh:417;
  * src/pf3d/kernels.cu:1071-1122): origin = rint(centre - (B-1)/2) and

@@ -49,6 +49,9 @@ struct Ctl {
 struct Slot {
     int n, cls;
     Ctl ctl;
+    // the cell's domain in a batched step: side, S pitch and the word
+    // offset of its replica's S in each slot arena
+    int L, P, s_off_lo, s_off_hi;
 };
 
 struct Smem {
@@ -57,6 +60,8 @@ struct Smem {
     int ired[kWaves * 8];
     Slot slot[2];
     uint32_t csraw[2][48];       // CellState (192 B) of the next cell
+    const pfh2d_replica* reps;   // batched step only: replica table
+    const uint16_t* cell_rep;    // batched step only: replica of each cell
 };
 
 // The staged window, an LDS object of its own: the global-to-LDS window
@@ -165,7 +170,11 @@ __device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_sta
     unsigned pctr = cs.promote_ctr;
     if (ex > 0) {
         if (ex + kPromoteSlack > WX || ey + kPromoteSlack > WY) {
-            const int grow = class_containing(ex, ey, kPromoteSlack);
+            int grow = class_containing(ex, ey, kPromoteSlack);
+            // a window must lie strictly inside the periodic domain: classes
+            // as wide as the side are unavailable (the batched step's
+            // 256-pixel replicas run without the 286-pixel fallback class)
+            if (grow >= 0 && max(shape_of(grow).wx, shape_of(grow).wy) >= A.L) grow = -1;
             if (grow >= 0) {
                 dcls = grow;
             } else if (CLS == kClassFallback && ex <= WX && ey <= WY) {
@@ -449,28 +458,43 @@ __device__ __forceinline__ void fetch4a(const pfh2d_step_args& A, Smem& sm, Fetc
 }
 
 // 4b (one thread, after 4a is visible to it): the per-cell scalars.
+template <bool kBatch>
 __device__ __forceinline__ void fetch4b(const pfh2d_step_args& A, Smem& sm, int which,
                                         unsigned long long step) {
     const int cls = sm.slot[which].cls;
-    if (cls >= 0 && cls < kNumClasses)
-        cell_scalars(scalar_args(A),
-                     reinterpret_cast<const pfh_cell_state*>(sm.csraw[which]), cls,
-                     step, &sm.slot[which].ctl);
+    if (cls >= 0 && cls < kNumClasses) {
+        ScalarArgs s = scalar_args(A);
+        if constexpr (kBatch) {
+            // batched step: the cell's replica sets its domain and stream
+            Slot& sl = sm.slot[which];
+            const pfh2d_replica& rp = sm.reps[sm.cell_rep[sl.n]];
+            s.L = rp.L;
+            s.polarity_seed = rp.polarity_seed;
+            sl.L = rp.L;
+            sl.P = rp.P;
+            sl.s_off_lo = (int)(uint32_t)rp.S_offset;
+            sl.s_off_hi = (int)(uint32_t)(rp.S_offset >> 32);
+        }
+        cell_scalars(s, reinterpret_cast<const pfh_cell_state*>(sm.csraw[which]), cls, step,
+                     &sm.slot[which].ctl);
+    }
 }
 
+template <bool kBatch>
 __device__ __forceinline__ void fetch4(const pfh2d_step_args& A, Smem& sm, Fetch& f,
                                        int which, unsigned long long step) {
     fetch4a(A, sm, f, which);
-    if (((int)threadIdx.x & (kWave - 1)) == kLeaderLane) fetch4b(A, sm, which, step);
+    if (((int)threadIdx.x & (kWave - 1)) == kLeaderLane) fetch4b<kBatch>(A, sm, which, step);
 }
 
+template <bool kBatch = false>
 __device__ __forceinline__ void fetch_all(const pfh2d_step_args& A, Smem& sm, int which,
                                           unsigned long long step) {
     Fetch f;
     fetch1(A, f);
     fetch2(A, f);
     fetch3(A, f);
-    fetch4(A, sm, f, which, step);
+    fetch4<kBatch>(A, sm, f, which, step);
 }
 
 // -------------------------------------------------------------------------
@@ -559,7 +583,25 @@ __device__ __forceinline__ void copy_rows0(const pfh2d_step_args& A, int nn, flo
 // kCheck: a check step (the surrogate's acceptance field), which needs the
 // new field and the cell's window position only: no scatter, moments,
 // support, flags or perimeter, and a commit of the position alone.
-template <int CLS, bool kCheck = false>
+// The cell's domain: the launch's own population or, in a batched step,
+// its replica's side, S pitch and S offset, made wave-uniform.
+struct Dom {
+    int L, P;
+    unsigned long long off;
+};
+template <bool kBatch>
+__device__ __forceinline__ Dom cell_dom(const pfh2d_step_args& A, const Slot& c) {
+    if constexpr (kBatch) {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)c.s_off_lo);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)c.s_off_hi);
+        return {__builtin_amdgcn_readfirstlane(c.L), __builtin_amdgcn_readfirstlane(c.P),
+                ((unsigned long long)hi << 32) | lo};
+    } else {
+        return {A.L, A.P, 0ull};
+    }
+}
+
+template <int CLS, bool kCheck = false, bool kBatch = false>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt,
                             int& pre, int& pslot) {
@@ -577,7 +619,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     const int y0 = band * RB;
     Fetch nf;
     if (control) fetch1(A, nf);
-    const int L = A.L, P = A.P;
+    const Dom dm = cell_dom<kBatch>(A, sm.slot[nxt ^ 1]);
+    const int L = dm.L, P = dm.P;
     // the origin from the LDS copy of the CellState: a global load here would
     // make the S loads wait for every load issued before it
     const int gx0 = c.gx0;
@@ -610,7 +653,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
 
     // Window rows of phi in flight (16 B each) ...
     const PFH_GLOBAL float* tin = gptr(A.phi_in) + (size_t)n * kTileArea;
-    const PFH_GLOBAL uint32_t* S_rd = gptr(A.S_rd);
+    const PFH_GLOBAL uint32_t* S_rd = gptr(A.S_rd + dm.off);
     const bool have = CLS == 0 && pre != 0;     // window already copied into LDS
     f32x4 ph[KCH];
     if (!have) {
@@ -784,7 +827,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
         // the aggregate's base in a VGPR pair: the per-row atomic then needs
         // neither the argument block nor a scalar wait
         const bool scatter = A.S_sc != nullptr;   // NULL: a step that scatters nothing
-        PFH_GLOBAL char* S_sc = reinterpret_cast<PFH_GLOBAL char*>(gptr(A.S_sc));
+        PFH_GLOBAL char* S_sc = reinterpret_cast<PFH_GLOBAL char*>(
+            gptr(scatter ? A.S_sc + dm.off : A.S_sc));
         asm volatile("" : "+v"(S_sc));
         uint32_t s_off = s_off0;
         const double ad = (double)a;
@@ -928,7 +972,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     }
     // The next cell's scalars, on a thread of wave 1: the first waves of each
     // SIMD finish phase B early, wave 0 (control lanes, commit) the latest.
-    if (tid == kScalarsThread) fetch4b(A, sm, nxt, step);
+    if (tid == kScalarsThread) fetch4b<kBatch>(A, sm, nxt, step);
     if constexpr (CLS == 0) {
         // piece 1 of the next window: each wave issues its rows as it
         // finishes phase B
@@ -959,14 +1003,17 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
 // Global-memory path: class 4 (window zeros synthesized) and the fallback
 // class 5 (the tile's own one-pixel zero ring), src/kernels.cu:604-937.
 // -------------------------------------------------------------------------
-template <int CLS>
+template <int CLS, bool kBatch = false>
 __device__ void cell_global(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt) {
     constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
     constexpr int TX0 = kShapes[CLS].tx0, TY0 = kShapes[CLS].ty0;
     const int tid = opaque_tid();
-    const int L = A.L, P = A.P;
-    if (tid < kControlLanes) fetch_all(A, sm, nxt, step);
+    const Dom dm = cell_dom<kBatch>(A, sm.slot[nxt ^ 1]);
+    const int L = dm.L, P = dm.P;
+    const uint32_t* S_rd = A.S_rd + dm.off;
+    uint32_t* S_sc = A.S_sc ? A.S_sc + dm.off : nullptr;
+    if (tid < kControlLanes) fetch_all<kBatch>(A, sm, nxt, step);
     const int gx0 = c.gx0, gy0 = c.gy0;
     const float* tin = A.phi_in + (size_t)n * kTileArea;
     float* tout = A.phi_out + (size_t)n * kTileArea;
@@ -985,7 +1032,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         const float cc = pin(x, y);
         const float gxv = 0.5f * (pin(x + 1, y) - pin(x - 1, y));
         const float gyv = 0.5f * (pin(x, y + 1) - pin(x, y - 1));
-        const uint32_t qS = A.S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
+        const uint32_t qS = S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
         const uint32_t qc = q_of(cc);
         float So = 0.0f;
         if (qS >= qc) So = (float)(qS - qc) * kQInv;
@@ -1011,7 +1058,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
                            + -20.0f * cC) * (float)(1.0 / 6.0);
         const float gxv = 0.5f * (cE - cW);
         const float gyv = 0.5f * (nC - sC);
-        const uint32_t qS = A.S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
+        const uint32_t qS = S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
         const uint32_t qc = q_of(cC);
         float So = 0.0f;
         if (qS >= qc) So = (float)(qS - qc) * kQInv;
@@ -1049,9 +1096,9 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         clampf |= (pn * pn > kQClampPhiSq);
         pmaxb = max(pmaxb, (int)__float_as_uint(fabsf(pn)));
         const uint32_t q = q_of(pn);
-        if (q && A.S_sc) {
+        if (q && S_sc) {
             const uint32_t old = atomicAdd(
-                &A.S_sc[(size_t)wrapi(gy0n + b, L) * P + wrapi(gx0n + a, L)], q);
+                &S_sc[(size_t)wrapi(gy0n + b, L) * P + wrapi(gx0n + a, L)], q);
             ovf |= old > 0xFFFFFFFFu - q;
         }
         const double d = (double)pn * (double)pn;
@@ -1081,22 +1128,30 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
 // Classes other than the round class 0 run out of line, with their own
 // register allocation, so their longer per-thread columns (up to 29 rows) do
 // not raise the pressure of the hot class-0 path.
-template <int CLS>
+template <int CLS, bool kBatch = false>
 __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
                                        unsigned long long step, Smem* sm, int cur) {
     if constexpr (CLS <= kClassStagedMax) {
         int none = 0, slot0 = 0;
-        cell_staged<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none, slot0);
+        cell_staged<CLS, false, kBatch>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none,
+                                        slot0);
     }
     else
-        cell_global<CLS>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
+        cell_global<CLS, kBatch>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
 }
 
 // -------------------------------------------------------------------------
 // The persistent step kernel (src/kernels.cu:948-1030).
 // -------------------------------------------------------------------------
-template <bool kCheck>
-__global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const int* run_if) {
+// Batched form (kBatch): A describes the whole ensemble as one population
+// (tiles, cell states, classes and the queue order over every replica's
+// cells; each S slot one arena holding every replica's aggregate; the
+// clear-ahead sweeps the whole arena), and each cell's side, S pitch, S
+// offset and polarity stream come from its replica's record.
+template <bool kCheck, bool kBatch = false>
+__global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const int* run_if,
+                                                 const pfh2d_replica* reps,
+                                                 const uint16_t* cell_rep) {
     __shared__ Smem sm;
     const int tid = (int)threadIdx.x;
     if (run_if != nullptr && *run_if == 0) return;   // gated-off launch: no effect
@@ -1121,12 +1176,18 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         *A.step_wr = *A.step_rd + 1ull;
     }
     const unsigned long long step = *A.step_rd;
-    if (tid == 0) sm.args = A;
+    if (tid == 0) {
+        sm.args = A;
+        if constexpr (kBatch) {
+            sm.reps = reps;
+            sm.cell_rep = cell_rep;
+        }
+    }
 
     // The control wavefront fetches each next cell one cell ahead (ticket,
     // id, class, CellState, scalars) while the other wavefronts compute, so
     // the queue's global round trips stay off the critical path.
-    if (tid < kControlLanes) fetch_all(A, sm, 0, step);
+    if (tid < kControlLanes) fetch_all<kBatch>(A, sm, 0, step);
     lds_barrier();
     int cur = 0;
     int pre = 0;                       // the current class-0 window is already in LDS
@@ -1137,15 +1198,16 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         if (cls < 0) break;
         switch (cls) {
             case 0:
-                cell_staged<0, kCheck>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre, pslot);
+                cell_staged<0, kCheck, kBatch>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre,
+                                               pslot);
                 break;
-            case 1: cell_cold<1>(&sm.args, n, step, &sm, cur); break;
-            case 2: cell_cold<2>(&sm.args, n, step, &sm, cur); break;
-            case 3: cell_cold<3>(&sm.args, n, step, &sm, cur); break;
-            case 4: cell_cold<4>(&sm.args, n, step, &sm, cur); break;
-            case 5: cell_cold<5>(&sm.args, n, step, &sm, cur); break;
+            case 1: cell_cold<1, kBatch>(&sm.args, n, step, &sm, cur); break;
+            case 2: cell_cold<2, kBatch>(&sm.args, n, step, &sm, cur); break;
+            case 3: cell_cold<3, kBatch>(&sm.args, n, step, &sm, cur); break;
+            case 4: cell_cold<4, kBatch>(&sm.args, n, step, &sm, cur); break;
+            case 5: cell_cold<5, kBatch>(&sm.args, n, step, &sm, cur); break;
             default:                       // unsupported id, flagged at fetch
-                if (tid < kControlLanes) fetch_all(A, sm, cur ^ 1, step);
+                if (tid < kControlLanes) fetch_all<kBatch>(A, sm, cur ^ 1, step);
                 lds_barrier();
                 break;
         }
@@ -1609,12 +1671,14 @@ int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* str
     // the step addresses S with 32-bit byte offsets
     if (4ull * (unsigned long long)a->P * (unsigned long long)a->L > 0xFFFFFFFFull)
         return invalid("aggregate larger than 4 GiB");
+    const pfh2d_replica* no_reps = nullptr;
+    const uint16_t* no_map = nullptr;
     if (check)
-        hipLaunchKernelGGL(pfh::k_step<true>, dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                           as_stream(stream), *a, run_if);
+        hipLaunchKernelGGL((pfh::k_step<true, false>), dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                           as_stream(stream), *a, run_if, no_reps, no_map);
     else
-        hipLaunchKernelGGL(pfh::k_step<false>, dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                           as_stream(stream), *a, run_if);
+        hipLaunchKernelGGL((pfh::k_step<false, false>), dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                           as_stream(stream), *a, run_if, no_reps, no_map);
     return launch_status("k_step");
 }
 }  // namespace
@@ -1640,15 +1704,38 @@ int pfh2d_launch_check_step_if(const pfh2d_step_args* a, const int* run_if, int 
     return step_launch(a, run_if, grid, stream, true);
 }
 
+int pfh2d_launch_batch_step(const pfh2d_step_args* a, const pfh2d_replica* replicas,
+                            const uint16_t* cell_replica, int num_replicas, int max_side,
+                            int max_pitch, int grid, void* stream) {
+    if (!a || !replicas || !cell_replica) return invalid("batch step: null argument");
+    if (a->moments) return invalid("moments must be NULL");
+    if (num_replicas <= 0 || num_replicas > 65536 || grid <= 0 || a->N <= 0 ||
+        !a->phi_in || !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||
+        !a->perm || !a->cursor_use || !a->cursor_clear || !a->step_rd || !a->step_wr ||
+        !a->flags || (a->clear_ahead_words && !a->S_cl) || a->parity_out < 0 ||
+        a->parity_out > 1)
+        return invalid("batch step args");
+    // every replica's side must exceed the classes it can use: 256 is the
+    // smallest side (class 4, 224 wide, still fits; the 286-wide fallback
+    // class is then unavailable and a cell that needs it is flagged)
+    if (max_side < 256 || max_pitch < max_side)
+        return invalid("batch step: replica sides must be at least 256");
+    if (4ull * (unsigned long long)max_pitch * (unsigned long long)max_side > 0xFFFFFFFFull)
+        return invalid("aggregate larger than 4 GiB");
+    hipLaunchKernelGGL((pfh::k_step<false, true>), dim3((unsigned)grid), dim3(pfh::kNT), 0,
+                       as_stream(stream), *a, (const int*)nullptr, replicas, cell_replica);
+    return launch_status("k_step (batched)");
+}
+
 int pfh2d_query_step_stats(int device, pfh_kernel_stats* out) {
     if (!out) return invalid("out");
     hipFuncAttributes fa{};
-    int st = hip_status(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pfh::k_step<false>)),
+    int st = hip_status(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pfh::k_step<false, false>)),
                         "hipFuncGetAttributes(k_step)");
     if (st != PFH_OK) return st;
     int blocks = 0;
     st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &blocks, reinterpret_cast<const void*>(pfh::k_step<false>), pfh::kNT, 0),
+                        &blocks, reinterpret_cast<const void*>(pfh::k_step<false, false>), pfh::kNT, 0),
                     "hipOccupancyMaxActiveBlocksPerMultiprocessor(k_step)");
     if (st != PFH_OK) return st;
     int max_threads = 0;

@@ -155,6 +155,7 @@ SIGNATURES = {
     "pfh2d_launch_step": (_I, [C.POINTER(StepArgs), _I, _P, _P, _SZ, _F, _I]),
     "pfh2d_launch_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
     "pfh2d_launch_check_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
+    "pfh2d_launch_batch_step": (_I, [C.POINTER(StepArgs), _P, _P, _I, _I, _I, _I, _P]),
     "pfh2d_query_step_stats": (_I, [_I, C.POINTER(KernelStats)]),
     "pfh2d_init_tiles": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _F, _F, _P]),
     "pfh2d_init_moments": (_I, [_P, _P, _P, _I, _P]),
@@ -180,6 +181,12 @@ SIGNATURES = {
     "pfh3d_finish_step": (_I, [C.POINTER(StepArgs3), _P]),
     "pfh3d_step": (_I, [C.POINTER(StepArgs3), _I, _P]),
     "pfh3d_pack_trajectory": (_I, [_P, _P, _I, _P]),
+    "pfh_crc64_ecma": (C.c_uint64, [C.c_uint64, _P, _SZ]),
+    "pfh3d_checkpoint_write": (_I, [C.c_char_p, C.c_int64, C.c_double, _P, _P, _I, _I, _P,
+                                    C.c_uint64, _P]),
+    "pfh3d_checkpoint_read": (_I, [C.c_char_p, _P, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _P, _P]),
     "pfh3d_update_slabs": (_I, [_I, _I, _I]),
     "pfh3d_brick_edge": (_I, [C.c_double, C.c_double]),
     "pfh3d_domain_side": (_I, [_I, C.c_double, C.c_double]),

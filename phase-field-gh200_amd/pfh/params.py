@@ -128,9 +128,11 @@ class SimParams:
         n = self.t_end / self.dt + 0.5
         return int(n) if 0.0 <= n <= 9.0e18 else -1
 
-    def validate(self) -> list[str]:
+    def validate(self, min_side: int = 287) -> list[str]:
         """Subset of validate() (include/params.cuh:596-722) that the kernels
-        rely on; returns the fatal messages (empty when valid)."""
+        rely on; returns the fatal messages (empty when valid). min_side is
+        the smallest side accepted: 287 (every class window fits), or 256
+        for a replica of a batched ensemble (no 286-wide fallback class)."""
         errs = []
         if not (self.dx == 1.0 and self.dy == 1.0):
             errs.append("dx and dy must both be exactly 1.0")
@@ -140,8 +142,9 @@ class SimParams:
             errs.append("domain side is invalid or exceeds the supported integer range")
         if self.num_cells <= 0:
             errs.append("num_cells must be positive")
-        if self.Nx > 0 and self.Nx <= 286:
-            errs.append(f"domain side {self.Nx} must exceed the largest rect dimension 286.")
+        if self.Nx > 0 and self.Nx < min_side:
+            errs.append(f"domain side {self.Nx} must exceed the largest rect dimension "
+                        f"{min_side - 1}.")
         gmax = max(self.gamma_normal, self.gamma_cancer)
         mrel = self.phase_field_mobility / PHASE_FIELD_M0
         if self.dt * gmax * mrel * LAP_ROW_SUM_BOUND >= 1.0:
@@ -221,3 +224,39 @@ def workload_c1() -> Workload2D:
 def workload_c2() -> Workload2D:
     """Periodic 2048^2, 1024 cells R=20 on a 32x32 grid (+-4 jitter), seed 7."""
     return monolayer(1024, 2048, 20.0, 32, 4.0, 7, name="pf2d_L2048_N1024_R20")
+
+
+ENSEMBLE_DOMAIN = 0x43344534
+ENSEMBLE_SIDES = (256, 512, 1024)
+
+
+def ensemble_max_cells(L: int, R: float = 20.0) -> int:
+    """Largest replica population: 90% of the lattice covered by disks of
+    radius R, floor(0.9 L^2 / (pi R^2))."""
+    return int(math.floor(0.9 * L * L / (K_PI * R * R)))
+
+
+def ensemble_replica(L: int, N: int, seed: int, R: float = 20.0) -> Workload2D:
+    """One replica of the batched ensemble: N cells of radius R on an L x L
+    periodic lattice, the reference's default grid + jitter placement and
+    Philox streams of `seed` (placement, polarity and tumbles)."""
+    rho = N * (K_PI * R * R) / float(L * L)
+    p = SimParams(Nx=L, Ny=L, num_cells=N, rho=rho, target_radius=R, seed=seed,
+                  polarity_seed=seed)
+    cx, cy = grid_jitter_centres(N, L, seed)
+    return Workload2D(f"replica_L{L}_N{N}_s{seed}", p, cx, cy)
+
+
+def ensemble_c4(replicas: int = 64, seed0: int = 100, R: float = 20.0) -> list[Workload2D]:
+    """The batched-ensemble workload: replica r (seed seed0 + r) has side L_r
+    uniform in {256, 512, 1024} and N_r uniform in [16, N_max(L_r)], both
+    drawn from Philox counter (r, 0x43344534, 0, 0) under key seed0 + r."""
+    out = []
+    for r in range(replicas):
+        seed = seed0 + r
+        v = philox4x32_10(r, ENSEMBLE_DOMAIN, 0, 0, seed & 0xFFFFFFFF, seed >> 32)
+        L = ENSEMBLE_SIDES[min(2, int(3.0 * uniform53(v[0], v[1])))]
+        nmax = ensemble_max_cells(L, R)
+        N = 16 + min(nmax - 16, int((nmax - 15) * uniform53(v[2], v[3])))
+        out.append(ensemble_replica(L, N, seed, R))
+    return out

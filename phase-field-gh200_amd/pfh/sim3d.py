@@ -67,6 +67,7 @@ class Sim3D:
         self.counters = torch.zeros(2, dtype=torch.int64, **z)
         self.partials = torch.zeros(N * self.shards * abi.PARTIAL3_BYTES, dtype=torch.uint8, **z)
         self.arrivals = torch.zeros(N, dtype=torch.int32, **z)
+        self.shards_alloc = self.shards
         self.seed_radius = seed_radius(p)
         self._args = [self._bake(s) for s in range(GRAPH_BODY)]
         self._graph = None
@@ -101,6 +102,16 @@ class Sim3D:
         a.polarity_stream = p.polarity_stream()
         a.aging_steps = p.aging_steps()
         return a
+
+    def set_launch_shape(self, shards: int, slabs: int) -> None:
+        """Measurement shards and update slabs of the following steps (the
+        shard partials were sized for the shard count at construction)."""
+        if not 1 <= shards <= self.shards_alloc or not 1 <= slabs <= self.B:
+            raise ValueError(f"shards must lie in [1, {self.shards_alloc}], slabs in "
+                             f"[1, {self.B}]")
+        self.shards, self.slabs = int(shards), int(slabs)
+        self._args = [self._bake(s) for s in range(GRAPH_BODY)]
+        self._graph = None
 
     def _sp(self):
         return C.c_void_p(self.stream.cuda_stream)
@@ -155,6 +166,58 @@ class Sim3D:
         self.flags.zero_()
         self.arrivals.zero_()
         torch.cuda.synchronize(self.dev)
+
+    # ---- checkpoints (PF3D schema 1) ----
+    def save_checkpoint(self, path, trajectory_interval: int = 0) -> None:
+        """The current state (accepted step, cell records, current cubes) as
+        a PF3D file; the shard count is recorded as the reduction contract."""
+        from . import checkpoint3d
+        self.sync()
+        step = int(self.counters[0].item())
+        rec = checkpoint3d.params_record(self.p, self.B, shards_policy=0,
+                                         trajectory_interval=trajectory_interval)
+        cells = self.cells()
+        abi.check(self.lib.pfh3d_checkpoint_write(
+            str(path).encode(), step, float(step) * self.p.dt, rec.ctypes.data,
+            cells.ctypes.data, self.N, self.B, _ptr(self.phi[self.steps_done % 2]),
+            self.shards, self._sp()), "pfh3d_checkpoint_write")
+
+    def load_checkpoint(self, path) -> None:
+        """Resume from a PF3D file written for this workload
+        (src/pf3d/sim.cu:840-880): cubes and records installed at the file's
+        step, the aggregate rebuilt from the cubes and the measurement
+        refreshed (no tumble)."""
+        from . import checkpoint3d
+        info = checkpoint3d.probe(path)
+        if info["N"] != self.N or info["B"] != self.B or int(info["params"]["Nx"]) != self.L:
+            raise ValueError(f"{path}: checkpoint geometry (N={info['N']}, B={info['B']}) "
+                             f"does not match this simulation (N={self.N}, B={self.B})")
+        step = int(info["step"])
+        slot = step % 2
+        cells = np.zeros(self.N, abi.CELL3_DTYPE)
+        pr = np.zeros(1, checkpoint3d.PARAMS3_DTYPE)
+        st, t, sh = C.c_int64(), C.c_double(), C.c_uint64()
+        n, b = C.c_int(), C.c_int()
+        self.sync()
+        abi.check(self.lib.pfh3d_checkpoint_read(
+            str(path).encode(), pr.ctypes.data, C.byref(st), C.byref(t), C.byref(sh),
+            C.byref(n), C.byref(b), cells.ctypes.data, _ptr(self.phi[slot]), self._sp()),
+            "pfh3d_checkpoint_read")
+        self.phi[1 - slot].zero_()
+        self.S[1 - slot].zero_()
+        self.cell.copy_(torch.from_numpy(cells.view(np.uint8).reshape(-1)))
+        self.counters.copy_(torch.tensor([step, step], dtype=torch.int64))
+        self.flags.zero_()
+        self.arrivals.zero_()
+        self.steps_done = step
+        lib, sp = self.lib, self._sp()
+        abi.check(lib.pfh3d_clear_S(_ptr(self.S[slot]), self.L, self.P, sp), "pfh3d_clear_S")
+        abi.check(lib.pfh3d_scatter_current(_ptr(self.phi[slot]), _ptr(self.cell),
+                                            _ptr(self.S[slot]), _ptr(self.flags), self.N, self.B,
+                                            self.L, self.P, sp), "pfh3d_scatter_current")
+        abi.check(lib.pfh3d_measure_cells(C.byref(self._args[slot]), 0, 1, 1, sp),
+                  "pfh3d_measure_cells")
+        self.sync()
 
     # ---- stepping (src/pf3d/sim.cu:1460-1651) ----
     def launch_one(self, slot: int) -> None:
