@@ -505,6 +505,15 @@ int pfh3d_finish_step(const pfh3d_step_args* a, void* stream);
  * finalize / restore / advance. Three launches, no host synchronisation, no
  * allocation: safe to capture into a HIP graph. */
 int pfh3d_step(const pfh3d_step_args* a, int slabs, void* stream);
+/* pfh3d_step that allocates nothing and takes no scratch from the caller:
+ * no counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). a->partials and a->arrivals are ignored; the shard
+ * partials and arrival words live in a library-owned array of 800 KiB per
+ * device (held until the process ends), so N <= 8192 and
+ * N * shards <= 8192, else PFH_ERR_INVALID. Every call on one device shares
+ * that array: calls that overlap on one device (concurrent streams) race. */
+int pfh3d_step_noalloc(const pfh3d_step_args* a, int slabs, void* stream);
 /* Replaces launch_pack_trajectory (kernels.cuh:482-485). */
 int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out,
                           int N, void* stream);

@@ -43,6 +43,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -989,7 +990,42 @@ __global__ __launch_bounds__(64) void k3_pack_traj(const pfh3d_cell_state* cells
 // -------------------------------------------------------------------------
 using namespace pfh_host;
 
+// Scratch of the allocation-free step: 8192 shard partials (96 B each)
+// followed by 8192 arrival words, 800 KiB per device, held until the
+// process ends. Every call on a device shares it.
+constexpr int kNoAllocMaxPartials = 8192;
+constexpr int kNoAllocMaxCells = 8192;
+constexpr size_t kNoAllocPartialBytes =
+    (size_t)kNoAllocMaxPartials * sizeof(pfh3d_moment_partial);
+constexpr size_t kNoAllocBytes = kNoAllocPartialBytes + (size_t)kNoAllocMaxCells * 4u;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, resolved on the first
+// call per device and kept.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    int st = hip_status(hipGetDevice(&dev), "hipGetDevice");
+    if (st != PFH_OK) return st;
+    if (dev < 0 || dev >= kMaxDevices) return invalid("device index");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        st = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                        "hipGetSymbolAddress(scratch)");
+        if (st != PFH_OK) return st;
+        if (!p) return hip_status(hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return PFH_OK;
+}
+
 
 int block_threads(int B) {
     const int g = B * B / 4;
@@ -1122,6 +1158,36 @@ int pfh3d_step(const pfh3d_step_args* a, int slabs, void* stream) {
     st = pfh3d_update(a, slabs, stream);
     if (st != PFH_OK) return st;
     return pfh3d_finish_step(a, stream);
+}
+
+// ---- allocation-free step ----
+// pfh3d_step with the shard partials and arrival words in the file-scope
+// array `scratch` instead of caller buffers. Host code never passes the
+// array's name to a kernel: it resolves the array's address on each device
+// once (hipGetSymbolAddress, with that device current) and passes that.
+int pfh3d_step_noalloc(const pfh3d_step_args* a, int slabs, void* stream) {
+    if (!a) return invalid("args");
+    if (a->N > kNoAllocMaxCells || (a->N > 0 && a->shards > 0 &&
+                                     (long long)a->N * a->shards > kNoAllocMaxPartials))
+        return invalid("step_noalloc: N must be <= 8192 and N * shards <= 8192 "
+                       "(the allocation-free scratch holds 8192 shard partials)");
+    void* base = nullptr;
+    int st = noalloc_scratch(&base);
+    if (st != PFH_OK) return st;
+    pfh3d_step_args b = *a;
+    b.partials = reinterpret_cast<pfh3d_moment_partial*>(base);
+    b.arrivals = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(base) +
+                                             kNoAllocPartialBytes);
+    if (b.N > 0) {
+        // the arrival words must start at zero: a fill kernel, so that the
+        // call can be captured in a graph
+        const size_t n4 = ((size_t)b.N + 3u) / 4u;
+        hipLaunchKernelGGL(pfh3::k3_clear, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0,
+                           as_stream(stream), b.arrivals, n4);
+        st = launch_status("k3_clear(arrivals)");
+        if (st != PFH_OK) return st;
+    }
+    return pfh3d_step(&b, slabs, stream);
 }
 
 int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out, int N,

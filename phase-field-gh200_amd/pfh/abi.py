@@ -180,6 +180,7 @@ SIGNATURES = {
     "pfh3d_update": (_I, [C.POINTER(StepArgs3), _I, _P]),
     "pfh3d_finish_step": (_I, [C.POINTER(StepArgs3), _P]),
     "pfh3d_step": (_I, [C.POINTER(StepArgs3), _I, _P]),
+    "pfh3d_step_noalloc": (_I, [C.POINTER(StepArgs3), _I, _P]),
     "pfh3d_pack_trajectory": (_I, [_P, _P, _I, _P]),
     "pfh_crc64_ecma": (C.c_uint64, [C.c_uint64, _P, _SZ]),
     "pfh3d_checkpoint_write": (_I, [C.c_char_p, C.c_int64, C.c_double, _P, _P, _I, _I, _P,
