@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
 BF16_PEAK_TFLOPS = 2500.0       # MI355X published dense BF16 MFMA rate
 TRAFFIC_SUMMARY = REPO / "profiles" / "pf2d_step_L2048_N1024_traffic.md"
 TRAFFIC_SUMMARY_3D = REPO / "profiles" / "pf3d_step_L256_N128_B56_traffic.md"
+TRAFFIC_SUMMARY_BATCHED = REPO / "profiles" / "pf2d_batched_64replicas_kernel_trace.md"
 
 
 def q_nonzero_count(sim) -> int:
@@ -199,6 +200,9 @@ def bench_3d(steps: int, warmup: int, with_cpu: bool) -> dict:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(TRAFFIC_SUMMARY_3D, "traffic_bytes_per_step"),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                                       "calibrated on a copy kernel, committed in profiles/"
+                                       + TRAFFIC_SUMMARY_3D.name + " (not measured in this run)",
                      "algorithmic_bytes_per_step": bytes_per_step,
                      "q_nonzero_per_step": Qbar},
     }
@@ -279,7 +283,12 @@ def bench_batched(steps: int, warmup: int) -> dict:
                    "sides": dict(Counter(ens.Ls)), "radius": 20.0, "seeds": "100..163",
                    "placement": "grid + jitter per replica", "graph_steps": 6},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(TRAFFIC_SUMMARY_BATCHED, "traffic_bytes_per_step"),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                                       "calibrated on a copy kernel, committed in profiles/"
+                                       + TRAFFIC_SUMMARY_BATCHED.name
+                                       + " (not measured in this run)",
                      "algorithmic_bytes_per_step": bytes_per_step,
                      "window_pixels_per_step": 0.5 * (W0 + W1),
                      "q_nonzero_per_step": 0.5 * (Q0 + Q1)},
@@ -316,8 +325,8 @@ def main() -> int:
     ap.add_argument("--steps-batched", type=int, default=120)
     ap.add_argument("--steps3d", type=int, default=600)
     ap.add_argument("--warmup3d", type=int, default=40)
-    ap.add_argument("--sur-blocks", type=int, default=25,
-                    help="8-step blocks of the surrogate A/B run (0 skips it)")
+    ap.add_argument("--sur-blocks", type=int, default=125,
+                    help="blocks of the surrogate A/B run (125 x 8 = 1000 steps; 0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 8 plain copies of MIB MiB "
                          "(the known-size kernel that scales hardware byte counters)")

@@ -416,6 +416,162 @@ static_assert(sizeof(pfh3d_traj_cell) == 104, "TrajPackedCell3D is 104 B");
 static_assert(sizeof(pfh3d_step_args) == 168, "step args layout");
 #endif
 
+/* ---- the reference's 3D launch records (include/pf3d/kernels.cuh:153-338,
+ * include/pf3d/rng.cuh:15-17, tile_partition.cuh:9-18, phase_storage.hpp:24)
+ * with the same member types and order, hence the same layout. ---- */
+typedef struct pfh3d_layout {          /* SLayout3D */
+    int nx, ny, nz;
+    int pitch_x;
+    uint32_t boundary_flags;           /* 7: periodic XYZ (the only one here) */
+} pfh3d_layout;
+typedef struct pfh3d_wall_field {      /* WeightedWallField3D */
+    const float* values;
+    int planes;
+} pfh3d_wall_field;
+typedef struct pfh3d_storage {         /* CellFieldStorage3D */
+    int z_cap;                         /* 0: cubic storage */
+} pfh3d_storage;
+typedef struct pfh3d_selection {       /* CellSelection3D: ids NULL = all */
+    const int* ids;
+    int count;
+} pfh3d_selection;
+typedef struct pfh3d_exchange_region { /* TileExchangeRegion3D */
+    int y_lo[4];
+    int y_hi[4];
+    int y_count;
+    int z_lo, z_hi;
+} pfh3d_exchange_region;
+typedef struct pfh3d_vec3 {            /* Vec3 */
+    double x, y, z;
+} pfh3d_vec3;
+typedef struct pfh3d_init_args {       /* InitArgs3D */
+    float* phi_even;
+    float* phi_odd;
+    pfh3d_cell_state* cells;
+    const pfh3d_vec3* centres;
+    int N;
+    int B;
+    pfh3d_layout layout;
+    float lambda;
+    float seed_radius;
+    pfh3d_storage storage;
+} pfh3d_init_args;
+typedef struct pfh3d_scatter_args {    /* ScatterArgs3D */
+    const float* phi;
+    pfh3d_cell_state* cells;
+    uint32_t* S;
+    uint32_t* global_flags;
+    pfh3d_layout layout;
+    int N;
+    int B;
+    pfh3d_selection selection;
+    pfh3d_storage storage;
+} pfh3d_scatter_args;
+typedef struct pfh3d_measure_args {    /* MeasureArgs3D */
+    const float* phi;
+    const uint32_t* S;
+    pfh3d_wall_field wall;
+    pfh3d_cell_state* cells;
+    uint32_t* global_flags;
+    const uint64_t* step;
+    pfh3d_layout layout;
+    int N;
+    int B;
+    pfh3d_moment_partial* partials;
+    int shards;
+    uint64_t polarity_stream;
+    uint64_t aging_steps;
+    double p_tumble;
+    float motility_coeff;
+    int max_shift;
+    int support_margin;
+    uint8_t apply_tumble;              /* C++ bool */
+    uint8_t compute_surface;           /* C++ bool */
+    pfh3d_selection selection;
+    pfh3d_storage storage;
+} pfh3d_measure_args;
+typedef struct pfh3d_update_args {     /* UpdateArgs3D */
+    float* phi_in;
+    float* phi_out;
+    const uint32_t* S_in;
+    uint32_t* S_out;
+    pfh3d_wall_field wall;
+    pfh3d_cell_state* cells;
+    uint32_t* global_flags;
+    unsigned long long* work_cursor;
+    pfh3d_layout layout;
+    int N;
+    int B;
+    float dt;
+    double V0;
+    double volume_scale;
+    float bulk_scale;
+    float interaction_coeff;
+    uint8_t compute_surface;           /* C++ bool */
+    pfh3d_selection selection;
+    int32_t tile_pass;                 /* UpdateTilePass3D: 0 = All */
+    pfh3d_exchange_region exchange_region;
+    pfh3d_storage storage;
+} pfh3d_update_args;
+typedef struct pfh3d_verify_cell {     /* VerifyCell3D */
+    double measured_V;
+    double state_V;
+    float max_abs_phi;
+    uint32_t nonfinite_count;
+    uint32_t support_edge_count;
+    uint32_t verified;
+} pfh3d_verify_cell;
+
+#ifndef __cplusplus
+#define PFH_SA _Static_assert
+#else
+#define PFH_SA static_assert
+#endif
+PFH_SA(sizeof(pfh3d_layout) == 20, "SLayout3D");
+PFH_SA(sizeof(pfh3d_init_args) == 72 && offsetof(pfh3d_init_args, layout) == 40 &&
+       offsetof(pfh3d_init_args, storage) == 68, "InitArgs3D");
+PFH_SA(sizeof(pfh3d_scatter_args) == 88 && offsetof(pfh3d_scatter_args, layout) == 32 &&
+       offsetof(pfh3d_scatter_args, selection) == 64, "ScatterArgs3D");
+PFH_SA(sizeof(pfh3d_measure_args) == 168 && offsetof(pfh3d_measure_args, layout) == 56 &&
+       offsetof(pfh3d_measure_args, partials) == 88 &&
+       offsetof(pfh3d_measure_args, polarity_stream) == 104 &&
+       offsetof(pfh3d_measure_args, apply_tumble) == 140 &&
+       offsetof(pfh3d_measure_args, selection) == 144, "MeasureArgs3D");
+PFH_SA(offsetof(pfh3d_update_args, layout) == 72 && offsetof(pfh3d_update_args, V0) == 104 &&
+       offsetof(pfh3d_update_args, compute_surface) == 128 &&
+       offsetof(pfh3d_update_args, selection) == 136 &&
+       offsetof(pfh3d_update_args, tile_pass) == 152 &&
+       offsetof(pfh3d_update_args, exchange_region) == 156 &&
+       sizeof(pfh3d_update_args) == 208, "UpdateArgs3D");
+PFH_SA(sizeof(pfh3d_verify_cell) == 32, "VerifyCell3D");
+#undef PFH_SA
+
+/* Replaces launch_initialize_spheres(const InitArgs3D&) (kernels.cuh:417). */
+int pfh3d_launch_initialize_spheres(const pfh3d_init_args* a, void* stream);
+/* Replaces launch_scatter_current(const ScatterArgs3D&) (kernels.cuh:420-421). */
+int pfh3d_launch_scatter_current(const pfh3d_scatter_args* a, void* stream);
+/* Replaces launch_measure_cells(const MeasureArgs3D&) and, with apply_motion,
+ * the following launch_apply_cell_motion (kernels.cuh:436-447). The
+ * reference finalizes the shard partials in a second kernel; here the last
+ * shard of each cell to finish does it, counting arrivals in `arrivals`
+ * (N words, zero between launches; may be NULL when shards == 1). dt is the
+ * step's binary32 dt (UpdateArgs3D::dt), read by the motion. */
+int pfh3d_launch_measure(const pfh3d_measure_args* a, uint32_t* arrivals, int apply_motion,
+                         float dt, void* stream);
+/* Replaces launch_update_tiled_fast / _sharded_fast / periodic_slab_walk_fast
+ * (const UpdateArgs3D&; kernels.cuh:506-533). S_out (zeroed by the caller)
+ * is required; slabs = z slabs per cell (0: automatic). */
+int pfh3d_launch_update(const pfh3d_update_args* a, int slabs, void* stream);
+/* Replaces launch_verify_cells (kernels.cuh:478-481): per cell, sum of
+ * phi^2 (fp64), max |phi|, non-finite voxels and voxels above 1e-5 on a
+ * face of the cube, and the stored V. */
+int pfh3d_verify_cells(const float* phi, const pfh3d_cell_state* cells, pfh3d_verify_cell* out,
+                       int N, int B, const pfh3d_layout* layout, void* stream);
+/* Replaces launch_verify_S (kernels.cuh:488-489): atomic max of the
+ * aggregate's words into *out_max (caller zeroes it). */
+int pfh3d_verify_S(const uint32_t* S, const pfh3d_layout* layout, uint32_t* out_max,
+                   void* stream);
+
 /* Parameter record of a PF3D schema-1 checkpoint: ParamsRecord3D
  * (common/checkpoint_format_3d.h:176-228), packed, 288 B, every offset as the
  * reference pins it. This build writes and reads the periodic-XYZ cube
@@ -622,6 +778,27 @@ int pfh_sur_writeback(float* phi, const pfh_cell_state* cells, const uint8_t* cl
 int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8_t* cls,
                   const int* origin, int N, int L, const float* pred, float tol, float* err,
                   uint32_t* rejected, uint32_t* err_max, const int* go, void* stream);
+/* Per-cell acceptance: for each cell, err[n] = relative L2 of the committed
+ * field (the prediction on the crop, zero elsewhere) against the check
+ * step's field over the crop and the check window; accept[n] = 1 when
+ * err <= tol, the error relative to the block's own increment (check field
+ * minus phi_n on the crop) <= tol_inc, and acc[n] + err <= budget, in which
+ * case acc[n] += err (the cell's accumulated committed error); *rejected
+ * counts the others; err_max (may be NULL) = running maximum of err. */
+int pfh_sur_check_cells(const float* phi_chk, const pfh_cell_state* cells_chk,
+                        const uint8_t* cls_chk, const float* phi_now,
+                        const pfh_cell_state* cells_now, const uint8_t* cls_now,
+                        const int* origin, int N, int L, const float* pred, float tol,
+                        float tol_inc, float budget, float* acc, float* err, int* accept,
+                        uint32_t* rejected, uint32_t* err_max, void* stream);
+/* Commit with per-cell fallback: accepted cells take the prediction in
+ * their window, the others the check step's field with its window position
+ * and class (cls_written[parity] and cls updated); q(phi) scattered into the
+ * zeroed aggregate S and every cell's moments recomputed. */
+int pfh_sur_commit_cells(float* phi, pfh_cell_state* cells, uint8_t* cls, const int* origin,
+                         const float* pred, const int* accept, const float* phi_chk,
+                         const pfh_cell_state* cells_chk, const uint8_t* cls_chk, int parity,
+                         int N, int L, int P, uint32_t* S, uint32_t* flags, void* stream);
 /* Accepted block: prediction into every cell's window, q(phi) scattered into
  * the zeroed aggregate S, moments as pfh2d_init_moments computes them.
  * Does nothing when skip_if is non-NULL and *skip_if != 0. */

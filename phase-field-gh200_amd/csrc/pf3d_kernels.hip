@@ -347,24 +347,26 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
     int lo0 = B, lo1 = B, lo2 = B, hi0 = -1, hi1 = -1, hi2 = -1;
     uint32_t pmax = 0u;
     const int wy = wrap_off(oy, y, L);
-    // S words of this lane's 4 voxels in plane z, fetched one plane ahead (used
-    // only where phi != 0, as before)
-    auto load_s = [&](int z, uint32_t (&sv)[4]) {
+    // S words of this lane's 4 voxels in plane z, fetched one plane ahead and
+    // only where that plane's phi (already in registers) is nonzero: the
+    // interaction reads S nowhere else, and most of a brick is empty
+    auto load_s = [&](int z, const f32x4& ph, uint32_t (&sv)[4]) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) sv[j] = 0u;
         if (act && z < ze) {
             const size_t srow = s_index(L, P, 0, wy, wrap_off(oz, z, L));
 #pragma unroll
-            for (int j = 0; j < 4; ++j) sv[j] = Sin[srow + (size_t)wrap_off(ox, x0 + j, L)];
+            for (int j = 0; j < 4; ++j)
+                if (ph[j] != 0.0f) sv[j] = Sin[srow + (size_t)wrap_off(ox, x0 + j, L)];
         }
     };
     uint32_t Sc[4], Sn[4];
-    load_s(zb, Sc);
+    load_s(zb, A0, Sc);
 
     for (int z = zb; z < ze; ++z) {
         const int cur = (z - zb) & 1;
         const f32x4 An = (z + 2 <= ze) ? load_plane(z + 2) : f32x4{0.f, 0.f, 0.f, 0.f};
-        load_s(z + 1, Sn);
+        load_s(z + 1, Ap, Sn);
         if (act) {
             const float* pl = lds + cur * PL;
             const f32x4 ym = *reinterpret_cast<const f32x4*>(pl + own - PX);

@@ -130,6 +130,11 @@ __device__ __forceinline__ int wrapi(int v, int L) {
     return v < 0 ? v + L : v;
 }
 
+// v in [0, 2L): a window coordinate (origin in [0, L), offset below the
+// widest class window, which is narrower than L) folded onto the domain
+// without a division.
+__device__ __forceinline__ int wrap_once(int v, int L) { return v >= L ? v - L : v; }
+
 // ---- Philox4x32-10 ----
 struct Philox4 { uint32_t v[4]; };
 

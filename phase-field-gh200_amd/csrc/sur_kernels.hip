@@ -548,9 +548,10 @@ __global__ __launch_bounds__(kIT, 1) void k_sur_infer(const InferArgs A) {
         for (int s = 0; s < 18; ++s)
             wa[s] = *reinterpret_cast<const bf16x8*>(wr + 16 * s + 8 * half);
     }
-    // wx: conv1 fragments (waves 0-5: k-steps 0-2) or the output layer's
-    // (waves 6-11: hi 0, lo 0, hi 1, lo 1); one array keeps both in one set of
-    // registers
+    // wx: conv1 fragments (waves 0-5: k-steps 0-2) or the output layer's two
+    // (waves 6-11: channels 0-15 and 16-31, each with the weights' bf16 high
+    // parts in rows 0-8 and low parts in rows 16-24); one array keeps both in
+    // one set of registers
     bf16x8 wx[3];
     if (early) {
 #pragma unroll
@@ -1021,6 +1022,106 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
     }
 }
 
+// Per-cell acceptance of a block (one workgroup of 256 threads per cell).
+// The committed field is the prediction on the crop and zero elsewhere, so
+// its error against the check field is measured over the crop and over the
+// check window outside the crop (a cell larger than the crop cannot pass):
+//   E = sum_crop (pred - chk)^2 + sum_{window \ crop} chk^2,  T = sum chk^2,
+//   I = sum_crop (chk - phi_n)^2  (the block's own increment).
+// A cell is accepted when sqrt(E/T) <= tol, sqrt(E/I) <= tol_inc (the
+// prediction carries the increment: predicting no change scores 1) and its
+// accumulated error acc[n] + sqrt(E/T) stays within budget; acc[n] grows by
+// the committed error of every accepted block. fp64 sums, fixed order.
+
+// Value of world pixel (wx, wy) in a cell's tile (0 outside its window).
+__device__ __forceinline__ float window_value(const float* tile, const pfh::Shape& sh, int gx0,
+                                              int gy0, int wx, int wy, int L) {
+    int a = wx - gx0;
+    if (a < 0) a += L;
+    int b = wy - gy0;
+    if (b < 0) b += L;
+    return (a < sh.wx && b < sh.wy)
+        ? tile[(size_t)(sh.ty0 + b) * pfh::kTilePitch + sh.tx0 + a] : 0.0f;
+}
+
+// One cell's decision, computed by a whole workgroup of NT threads.
+template <int NT>
+__device__ void decide_cell(
+    int n, const float* phi_chk, const pfh_cell_state* cells_chk, const uint8_t* cls_chk,
+    const float* phi_now, const pfh_cell_state* cells_now, const uint8_t* cls_now,
+    const int* origin, int L, const float* pred, float tol, float tol_inc, float budget,
+    float* acc, float* err, int* accept, uint32_t* rejected, uint32_t* err_max) {
+    constexpr int NW = NT / 64;
+    __shared__ double red[3][NW];
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    const pfh::Shape sc = pfh::shape_of((int)cls_chk[n]);
+    const pfh::Shape sn = pfh::shape_of((int)cls_now[n]);
+    const int gxc = cells_chk[n].gx0, gyc = cells_chk[n].gy0;
+    const int gxn = cells_now[n].gx0, gyn = cells_now[n].gy0;
+    const float* tc = phi_chk + (size_t)n * pfh::kTileArea;
+    const float* tn = phi_now + (size_t)n * pfh::kTileArea;
+    const float* pr = pred + (size_t)n * kCropPix;
+    double e = 0.0, t = 0.0, inc = 0.0;
+    for (int p = (int)threadIdx.x; p < kCropPix; p += NT) {
+        const int j = p / kCrop, i = p - j * kCrop;
+        int wx = ox + i, wy = oy + j;
+        if (wx >= L) wx -= L;
+        if (wy >= L) wy -= L;
+        const double c = (double)window_value(tc, sc, gxc, gyc, wx, wy, L);
+        const double v = (double)window_value(tn, sn, gxn, gyn, wx, wy, L);
+        const double d = (double)pr[p] - c;
+        e += d * d;
+        t += c * c;
+        inc += (c - v) * (c - v);
+    }
+    // gxc, ox in [0, L) and a < wx < L, so i lies in (-L, 2L): one wrap
+    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += NT) {
+        const int b = p / sc.wx, a = p - b * sc.wx;
+        int i = gxc + a - ox, j = gyc + b - oy;
+        i += i < 0 ? L : (i >= L ? -L : 0);
+        j += j < 0 ? L : (j >= L ? -L : 0);
+        if (i < kCrop && j < kCrop) continue;          // counted on the crop
+        const double c = (double)tc[(size_t)(sc.ty0 + b) * pfh::kTilePitch + sc.tx0 + a];
+        e += c * c;
+        t += c * c;
+    }
+    e = pfh::wave_sum(e);
+    t = pfh::wave_sum(t);
+    inc = pfh::wave_sum(inc);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = e;
+        red[1][threadIdx.x >> 6] = t;
+        red[2][threadIdx.x >> 6] = inc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double E = 0.0, T = 0.0, I = 0.0;
+        for (int w = 0; w < NW; ++w) {
+            E += red[0][w];
+            T += red[1][w];
+            I += red[2][w];
+        }
+        const float r = T > 0.0 ? (float)sqrt(E / T) : (E > 0.0 ? 1e30f : 0.0f);
+        const float ri = I > 0.0 ? (float)sqrt(E / I) : (E > 0.0 ? 1e30f : 0.0f);
+        const bool ok = r <= tol && ri <= tol_inc && acc[n] + r <= budget;
+        err[n] = r;
+        accept[n] = ok ? 1 : 0;
+        if (ok) acc[n] += r;
+        else atomicAdd(rejected, 1u);
+        if (err_max) atomicMax(err_max, __float_as_uint(r));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sur_check_cells(
+    const float* phi_chk, const pfh_cell_state* cells_chk, const uint8_t* cls_chk,
+    const float* phi_now, const pfh_cell_state* cells_now, const uint8_t* cls_now,
+    const int* origin, int L, const float* pred, float tol, float tol_inc, float budget,
+    float* acc, float* err, int* accept, uint32_t* rejected, uint32_t* err_max) {
+    decide_cell<256>((int)blockIdx.x, phi_chk, cells_chk, cls_chk, phi_now, cells_now, cls_now,
+                     origin, L, pred, tol, tol_inc, budget, acc, err, accept, rejected,
+                     err_max);
+}
+
 // Accepted block, one workgroup of 1024 threads per cell: writes the
 // prediction into the tile's active window (window pixels outside the crop
 // become 0), scatters q(phi) into the zeroed aggregate S and stores the
@@ -1033,13 +1134,13 @@ __global__ __launch_bounds__(256) void k_sur_check(const float* phi_chk,
 template <int CLS>
 __device__ void commit_staged(float* t, const float* pr, int L, int P, uint32_t* S, int gx0,
                               int gy0, int ox, int oy, pfh::MomentPart& mp, bool& ovf,
-                              bool& clampf) {
+                              bool& clampf, const float* src = nullptr) {
     constexpr int WX = pfh::kShapes[CLS].wx, WY = pfh::kShapes[CLS].wy;
     constexpr int NB = pfh::kPixThreads / WX, RB = (WY + NB - 1) / NB;
     const int pix = (int)threadIdx.x - pfh::kControlLanes;
     if (pix < 0 || pix / WX >= NB) return;
     const int x = pix % WX, y0 = (pix / WX) * RB;
-    const int wx = pfh::wrapi(gx0 + x, L);
+    const int wx = pfh::wrap_once(gx0 + x, L);
     int i = wx - ox;
     if (i < 0) i += L;
     const bool icrop = i < kCrop;
@@ -1055,10 +1156,12 @@ __device__ void commit_staged(float* t, const float* pr, int L, int P, uint32_t*
             olds[k] = 0u;
             const int y = y0 + k0 + k;
             if (k0 + k < RB && y < WY) {
-                const int wy = pfh::wrapi(gy0 + y, L);
+                const int wy = pfh::wrap_once(gy0 + y, L);
                 int j = wy - oy;
                 if (j < 0) j += L;
-                const float v = (icrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
+                // the prediction on the crop, or (src) a whole window's field
+                const float v = src ? src[(size_t)y * pfh::kTilePitch + x]
+                                    : ((icrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f);
                 t[(size_t)y * pfh::kTilePitch + x] = v;
                 const uint32_t q = pfh::q_of(v);
                 if (q) {
@@ -1102,7 +1205,7 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
         m = pfh::reduce_moments(mp, sc.wx, sc.wy);
     } else {
         m = pfh::window_moments(k, [&](int a, int b) {
-            const int wx = pfh::wrapi(gx0 + a, L), wy = pfh::wrapi(gy0 + b, L);
+            const int wx = pfh::wrap_once(gx0 + a, L), wy = pfh::wrap_once(gy0 + b, L);
             int i = wx - ox;
             if (i < 0) i += L;
             int j = wy - oy;
@@ -1121,6 +1224,90 @@ __global__ __launch_bounds__(pfh::kNT) void k_sur_commit(float* phi, pfh_cell_st
     if (ovf) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
     if (clampf) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
     if (threadIdx.x == 0) pfh::store_moments(&cells[n], m);
+}
+
+// A block with per-cell fallback: an accepted cell (accept[n] != 0) takes
+// the prediction in its current window; a rejected cell takes the check
+// step's field (the plain stencil's own step of K dt) with the check step's
+// window position and class. Either way the field is written, q(phi) is
+// scattered into the zeroed aggregate and the moments are recomputed as in
+// k_sur_commit.
+__device__ void commit_cell(
+    int n, bool take, float* phi, pfh_cell_state* cells, uint8_t* cls, const int* origin,
+    const float* pred, const float* phi_chk, const pfh_cell_state* cells_chk,
+    const uint8_t* cls_chk, int parity, int L, int P, uint32_t* S, uint32_t* flags) {
+    const int kold = (int)cls[n];
+    const int k = take ? kold : (int)cls_chk[n];
+    const pfh::Shape sc = pfh::shape_of(k);
+    const int gx0 = take ? cells[n].gx0 : cells_chk[n].gx0;
+    const int gy0 = take ? cells[n].gy0 : cells_chk[n].gy0;
+    const int ox = origin[2 * n], oy = origin[2 * n + 1];
+    float* tile = phi + (size_t)n * pfh::kTileArea;
+    if (k != kold) {
+        // the window moves to another class layout: clear the tile first
+        for (int i = (int)threadIdx.x; i < pfh::kTileArea; i += pfh::kNT) tile[i] = 0.0f;
+        __syncthreads();
+    }
+    float* t = tile + (size_t)sc.ty0 * pfh::kTilePitch + sc.tx0;
+    const float* src = take ? nullptr
+        : phi_chk + (size_t)n * pfh::kTileArea + (size_t)sc.ty0 * pfh::kTilePitch + sc.tx0;
+    const float* pr = pred + (size_t)n * kCropPix;
+    bool ovf = false, clampf = false;
+    pfh::WindowMoments m;
+    if (k <= pfh::kClassStagedMax) {
+        pfh::MomentPart mp = {0.0, 0.0, 0.0, sc.wx, -1, sc.wy, -1, 0};
+        switch (k) {
+            case 0: commit_staged<0>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf, src); break;
+            case 1: commit_staged<1>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf, src); break;
+            case 2: commit_staged<2>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf, src); break;
+            default: commit_staged<3>(t, pr, L, P, S, gx0, gy0, ox, oy, mp, ovf, clampf, src); break;
+        }
+        m = pfh::reduce_moments(mp, sc.wx, sc.wy);
+    } else {
+        m = pfh::window_moments(k, [&](int a, int b) {
+            const int wx = pfh::wrap_once(gx0 + a, L), wy = pfh::wrap_once(gy0 + b, L);
+            float v;
+            if (src) {
+                v = src[(size_t)b * pfh::kTilePitch + a];
+            } else {
+                int i = wx - ox;
+                if (i < 0) i += L;
+                int j = wy - oy;
+                if (j < 0) j += L;
+                v = (i < kCrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f;
+            }
+            t[(size_t)b * pfh::kTilePitch + a] = v;
+            const uint32_t q = pfh::q_of(v);
+            if (q) {
+                const uint32_t old = atomicAdd(&S[(size_t)wy * P + wx], q);
+                ovf |= old > 0xFFFFFFFFu - q;
+                clampf |= v * v > pfh::kQClampPhiSq;
+            }
+            return v;
+        });
+    }
+    if (ovf) atomicOr(&flags[PFH_FLAG_S_OVERFLOW], 1u);
+    if (clampf) atomicOr(&flags[PFH_FLAG_Q_CLAMP], 1u);
+    if (threadIdx.x == 0) {
+        pfh_cell_state* cs = &cells[n];
+        pfh::store_moments(cs, m);
+        if (!take) {
+            cs->gx0 = gx0;
+            cs->gy0 = gy0;
+            cs->cls = (uint8_t)k;
+            cs->cls_written[parity] = (uint8_t)k;
+            cls[n] = (uint8_t)k;
+        }
+    }
+}
+
+__global__ __launch_bounds__(pfh::kNT) void k_sur_commit_cells(
+    float* phi, pfh_cell_state* cells, uint8_t* cls, const int* origin, const float* pred,
+    const int* accept, const float* phi_chk, const pfh_cell_state* cells_chk,
+    const uint8_t* cls_chk, int parity, int L, int P, uint32_t* S, uint32_t* flags) {
+    const int n = (int)blockIdx.x;
+    commit_cell(n, accept[n] != 0, phi, cells, cls, origin, pred, phi_chk, cells_chk, cls_chk,
+                parity, L, P, S, flags);
 }
 
 // The polarity of K skipped steps: the plain step's tumble draw
@@ -1454,6 +1641,36 @@ int pfh_sur_check(const float* phi_chk, const pfh_cell_state* cells, const uint8
     hipLaunchKernelGGL(sur::k_sur_check, dim3((unsigned)N), dim3(256), 0, as_stream(stream),
                        phi_chk, cells, cls, origin, L, pred, tol, err, rejected, err_max, go);
     return launch_status("k_sur_check");
+}
+
+int pfh_sur_check_cells(const float* phi_chk, const pfh_cell_state* cells_chk,
+                        const uint8_t* cls_chk, const float* phi_now,
+                        const pfh_cell_state* cells_now, const uint8_t* cls_now,
+                        const int* origin, int N, int L, const float* pred, float tol,
+                        float tol_inc, float budget, float* acc, float* err, int* accept,
+                        uint32_t* rejected, uint32_t* err_max, void* stream) {
+    if (!phi_chk || !cells_chk || !cls_chk || !phi_now || !cells_now || !cls_now || !origin ||
+        !pred || !acc || !err || !accept || !rejected || N <= 0 || L < sur::kCrop)
+        return invalid("sur_check_cells");
+    hipLaunchKernelGGL(sur::k_sur_check_cells, dim3((unsigned)N), dim3(256), 0,
+                       as_stream(stream), phi_chk, cells_chk, cls_chk, phi_now, cells_now,
+                       cls_now, origin, L, pred, tol, tol_inc, budget, acc, err, accept,
+                       rejected, err_max);
+    return launch_status("k_sur_check_cells");
+}
+
+int pfh_sur_commit_cells(float* phi, pfh_cell_state* cells, uint8_t* cls, const int* origin,
+                         const float* pred, const int* accept, const float* phi_chk,
+                         const pfh_cell_state* cells_chk, const uint8_t* cls_chk, int parity,
+                         int N, int L, int P, uint32_t* S, uint32_t* flags, void* stream) {
+    if (!phi || !cells || !cls || !origin || !pred || !accept || !phi_chk || !cells_chk ||
+        !cls_chk || !S || !flags || N <= 0 || L < sur::kCrop || P < L || parity < 0 ||
+        parity > 1)
+        return invalid("sur_commit_cells");
+    hipLaunchKernelGGL(sur::k_sur_commit_cells, dim3((unsigned)N), dim3(pfh::kNT), 0,
+                       as_stream(stream), phi, cells, cls, origin, pred, accept, phi_chk,
+                       cells_chk, cls_chk, parity, L, P, S, flags);
+    return launch_status("k_sur_commit_cells");
 }
 
 int pfh_sur_commit(float* phi, pfh_cell_state* cells, const uint8_t* cls, const int* origin,

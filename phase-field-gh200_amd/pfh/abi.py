@@ -137,6 +137,72 @@ class StepArgs3(C.Structure):
 
 
 assert C.sizeof(StepArgs3) == 168, C.sizeof(StepArgs3)
+
+
+# The reference's 3D launch records (include/pf3d/kernels.cuh:153-338).
+class Layout3(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("pitch_x", C.c_int),
+                ("boundary_flags", C.c_uint32)]
+
+
+class WallField3(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("planes", C.c_int)]
+
+
+class Storage3(C.Structure):
+    _fields_ = [("z_cap", C.c_int)]
+
+
+class Selection3(C.Structure):
+    _fields_ = [("ids", C.c_void_p), ("count", C.c_int)]
+
+
+class ExchangeRegion3(C.Structure):
+    _fields_ = [("y_lo", C.c_int * 4), ("y_hi", C.c_int * 4), ("y_count", C.c_int),
+                ("z_lo", C.c_int), ("z_hi", C.c_int)]
+
+
+class InitArgs3(C.Structure):
+    _fields_ = [("phi_even", C.c_void_p), ("phi_odd", C.c_void_p), ("cells", C.c_void_p),
+                ("centres", C.c_void_p), ("N", C.c_int), ("B", C.c_int), ("layout", Layout3),
+                ("lambda_", C.c_float), ("seed_radius", C.c_float), ("storage", Storage3)]
+
+
+class ScatterArgs3(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("cells", C.c_void_p), ("S", C.c_void_p),
+                ("global_flags", C.c_void_p), ("layout", Layout3), ("N", C.c_int),
+                ("B", C.c_int), ("selection", Selection3), ("storage", Storage3)]
+
+
+class MeasureArgs3(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("S", C.c_void_p), ("wall", WallField3),
+                ("cells", C.c_void_p), ("global_flags", C.c_void_p), ("step", C.c_void_p),
+                ("layout", Layout3), ("N", C.c_int), ("B", C.c_int), ("partials", C.c_void_p),
+                ("shards", C.c_int), ("polarity_stream", C.c_uint64),
+                ("aging_steps", C.c_uint64), ("p_tumble", C.c_double),
+                ("motility_coeff", C.c_float), ("max_shift", C.c_int),
+                ("support_margin", C.c_int), ("apply_tumble", C.c_uint8),
+                ("compute_surface", C.c_uint8), ("selection", Selection3),
+                ("storage", Storage3)]
+
+
+class UpdateArgs3(C.Structure):
+    _fields_ = [("phi_in", C.c_void_p), ("phi_out", C.c_void_p), ("S_in", C.c_void_p),
+                ("S_out", C.c_void_p), ("wall", WallField3), ("cells", C.c_void_p),
+                ("global_flags", C.c_void_p), ("work_cursor", C.c_void_p),
+                ("layout", Layout3), ("N", C.c_int), ("B", C.c_int), ("dt", C.c_float),
+                ("V0", C.c_double), ("volume_scale", C.c_double),
+                ("bulk_scale", C.c_float), ("interaction_coeff", C.c_float),
+                ("compute_surface", C.c_uint8), ("selection", Selection3),
+                ("tile_pass", C.c_int), ("exchange_region", ExchangeRegion3),
+                ("storage", Storage3)]
+
+
+VERIFY3_DTYPE = np.dtype([("measured_V", "<f8"), ("state_V", "<f8"), ("max_abs_phi", "<f4"),
+                          ("nonfinite_count", "<u4"), ("support_edge_count", "<u4"),
+                          ("verified", "<u4")])
+assert (C.sizeof(InitArgs3), C.sizeof(ScatterArgs3), C.sizeof(MeasureArgs3),
+        C.sizeof(UpdateArgs3), VERIFY3_DTYPE.itemsize) == (72, 88, 168, 208, 32)
 assert TRAJ3_DTYPE.itemsize == 104
 
 _P = C.c_void_p
@@ -182,6 +248,12 @@ SIGNATURES = {
     "pfh3d_step": (_I, [C.POINTER(StepArgs3), _I, _P]),
     "pfh3d_step_noalloc": (_I, [C.POINTER(StepArgs3), _I, _P]),
     "pfh3d_pack_trajectory": (_I, [_P, _P, _I, _P]),
+    "pfh3d_launch_initialize_spheres": (_I, [C.POINTER(InitArgs3), _P]),
+    "pfh3d_launch_scatter_current": (_I, [C.POINTER(ScatterArgs3), _P]),
+    "pfh3d_launch_measure": (_I, [C.POINTER(MeasureArgs3), _P, _I, _F, _P]),
+    "pfh3d_launch_update": (_I, [C.POINTER(UpdateArgs3), _I, _P]),
+    "pfh3d_verify_cells": (_I, [_P, _P, _P, _I, _I, C.POINTER(Layout3), _P]),
+    "pfh3d_verify_S": (_I, [_P, C.POINTER(Layout3), _P, _P]),
     "pfh_crc64_ecma": (C.c_uint64, [C.c_uint64, _P, _SZ]),
     "pfh3d_checkpoint_write": (_I, [C.c_char_p, C.c_int64, C.c_double, _P, _P, _I, _I, _P,
                                     C.c_uint64, _P]),
@@ -201,6 +273,10 @@ SIGNATURES = {
     "pfh_sur_check": (_I, [_P, _P, _P, _P, _I, _I, _P, _F, _P, _P, _P, _P, _P]),
     "pfh_sur_commit": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "pfh_sur_zero2": (_I, [_P, _P, C.c_ulonglong, _P, _P]),
+    "pfh_sur_check_cells": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _F, _F, _F, _P, _P,
+                                 _P, _P, _P, _P]),
+    "pfh_sur_commit_cells": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P,
+                                  _P]),
     "pfh_sur_advance": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P]),
     "pfh_sur_prep": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, C.c_ulonglong, _P, _P, _P, _P]),
     "pfh_sur_round_bf16": (_I, [_P, _P, _I, _P]),

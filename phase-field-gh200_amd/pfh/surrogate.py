@@ -12,13 +12,19 @@ backward kernels.
 
 Acceptance (the check-step rule of this build): for each cell, one explicit
 plain step of size K dt from phi_n (the plain kernel with dt' = K dt, run on
-copies of the state) gives a first-order reference of phi_{n+K}; a cell is
-accepted when the relative L2 distance between the prediction and that
-reference on its crop is at most `tol`. If every cell is accepted the
-predictions are written into the tiles and the aggregate S, the moments and
-the K skipped polarity draws are recomputed exactly (k_scatter_all /
-k_init_moments / the plain tumble rule); otherwise the block is advanced by
-K plain steps.
+copies of the state) gives a first-order reference of phi_{n+K}. Per-cell
+rule (the default): a cell is accepted when, over its whole window, the
+relative L2 distance E between the prediction and that reference is at most
+`tol`, the distance relative to the block's own increment (reference -
+phi_n) is at most `tol_inc` (a prediction of no change scores 1), and the
+cell's accumulated committed error (the sum of E over its accepted blocks)
+stays within `budget`. An accepted cell takes the prediction; a rejected
+cell takes the reference field with the reference's window position and
+class. The aggregate S, the moments and the K skipped polarity draws are
+then recomputed exactly from the written fields (k_scatter_all /
+k_init_moments / the plain tumble rule). Population rule
+(fallback="population"): every cell must pass `tol` on its crop, otherwise
+the whole block is advanced by K plain steps.
 """
 from __future__ import annotations
 
@@ -343,10 +349,27 @@ def deploy_weights(net: Net) -> tuple[torch.Tensor, float]:
 # ---------------------------------------------------------------------------
 # deployment on a Sim2D
 # ---------------------------------------------------------------------------
+def check_step_stable(p, K: int) -> bool:
+    """Whether one explicit step of K dt is itself a stable integrator, so
+    that a rejected cell may take it: the largest decay rate of the
+    linearised update (the 9-point Laplacian's 32/6 gamma, the double well's
+    bulk gamma at phi = 0 or 1, the interaction's rep S_other with
+    S_other <= 1, each scaled by the mobility ratio) times K dt stays below
+    2. At dt = 0.01 with the monolayer's coefficients this holds for K <= 11
+    (K = 8: 1.46; K = 16: 2.9, where a fallback to the check field grows
+    without bound)."""
+    gam = max(p.gamma_normal, p.gamma_cancer if p.cancer_fraction > 0 else 0.0)
+    mrel = p.phase_field_mobility / 0.5
+    rate = mrel * (gam * 32.0 / 6.0 + p.bulk() * gam + p.interaction())
+    return K * p.dt * rate < 2.0
+
+
 class SurrogateRunner:
     """Advances a Sim2D K steps at a time with the surrogate, checked per
-    cell against one explicit step of size K dt; a block with any rejected
-    cell is replaced by K plain steps (whole-population fallback).
+    cell against one explicit step of size K dt. Per-cell fallback (the
+    default): a rejected cell takes that check step's field, so a block
+    always advances. Population fallback: a block with any rejected cell is
+    replaced by K plain steps.
 
     Blocks are issued in chunks with no host round trip inside a chunk. Every
     launch of a block is gated on a device flag `go`: propose (check-step
@@ -358,13 +381,32 @@ class SurrogateRunner:
     chunk the host runs the K plain steps of that block, sets `go` again and
     issues the remaining blocks. With `graphs` the launches of a block are
     captured once per residue of the step index mod 6 (the phi parity and S
-    rotation it touches) and replayed."""
+    rotation it touches) and replayed. In per-cell mode the rejected count
+    stays zero and no chunk is ever interrupted."""
 
-    def __init__(self, sim, net: Net, K: int = 8, tol: float = 1e-3, graphs: bool = True,
-                 chunk: int = 8):
+    def __init__(self, sim, net: Net, K: int | None = None, tol: float = 1e-3,
+                 graphs: bool = True, chunk: int = 8, fallback: str | None = None,
+                 tol_inc: float = 0.25, budget: float = 5e-4):
+        K = net.K if K is None else int(K)
+        if K != net.K:
+            raise ValueError(f"the weights predict {net.K} steps; K = {K} would apply them "
+                             f"as {K}")
         if K <= 0 or K % 2:
             raise ValueError("K must be a positive even number of steps")
+        stable = check_step_stable(sim.p, K)
+        if fallback is None:
+            fallback = "cell" if stable else "population"
+        if fallback not in ("cell", "population"):
+            raise ValueError("fallback is 'cell' or 'population'")
+        if fallback == "cell" and not stable:
+            raise ValueError(f"a rejected cell would take one explicit step of {K} dt, which "
+                             f"is not a stable integrator for these coefficients; use "
+                             f"fallback='population'")
         self.sim, self.net, self.K, self.tol = sim, net, K, tol
+        # per-cell fallback: a rejected cell takes the check step's field;
+        # acceptance also bounds the error relative to the block's increment
+        # and each cell's accumulated committed error (budget)
+        self.fallback, self.tol_inc, self.budget = fallback, tol_inc, budget
         self.chunk = max(1, int(chunk))
         self.lib = abi.load()
         dev = sim.dev
@@ -382,7 +424,10 @@ class SurrogateRunner:
         self.ctl = torch.tensor([1, 0], dtype=torch.int32, device=dev)
         self.abort_step = torch.zeros(1, dtype=torch.int64, device=dev)
         # [0] largest check error so far (float bits), [1] rejected blocks
-        self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
+        # [2] cells that fell back (per-cell mode), summed over blocks
+        self.counters = torch.zeros(3, dtype=torch.int32, device=dev)
+        self.acc = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.accept = torch.zeros(N, dtype=torch.int32, device=dev)
         self.graphs = graphs
         self._g = {}
         self.blocks = 0
@@ -424,6 +469,28 @@ class SurrogateRunner:
                     go=self.ctl)
         a = self._check_args(s)
         abi.check(lib.pfh2d_launch_check_step_if(C.byref(a), go, sim.grid, sp), "check step")
+        if self.fallback == "cell":
+            # every cell is decided on its own; rejected cells take the check
+            # step's field, so the block always advances (rej stays 0)
+            abi.check(lib.pfh_sur_zero2(sim._s_ptr(e % 3), sim._s_ptr((e + 1) % 3), sim.words,
+                                        None, sp), "zero2")
+            abi.check(lib.pfh_sur_check_cells(
+                _ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk), _ptr(sim.phi[pin]),
+                _ptr(sim.cell), _ptr(sim.cls), _ptr(self.ws.origin), N, sim.L,
+                _ptr(self.ws.pred), self.tol, self.tol_inc, self.budget, _ptr(self.acc),
+                _ptr(self.err), _ptr(self.accept), _ptr(self.counters) + 8,
+                _ptr(self.counters), sp), "check cells")
+            abi.check(lib.pfh_sur_commit_cells(
+                _ptr(sim.phi[pin]), _ptr(sim.cell), _ptr(sim.cls), _ptr(self.ws.origin),
+                _ptr(self.ws.pred), _ptr(self.accept), _ptr(self.phi_chk), _ptr(self.cell_chk),
+                _ptr(self.cls_chk), pin, N, sim.L, sim.P, sim._s_ptr(e % 3), _ptr(sim.flags),
+                sp), "commit cells")
+            abi.check(lib.pfh_sur_finish(_ptr(sim.cell), N, _ptr(self.step_chk), K,
+                                         sim.p.polarity_stream(), sim.p.p_tumble(),
+                                         _ptr(sim.stepc), _ptr(sim.cursor), pin, go, rej,
+                                         _ptr(self.abort_step), _ptr(self.counters) + 4, sp),
+                      "finish")
+            return
         abi.check(lib.pfh_sur_check(_ptr(self.phi_chk), _ptr(self.cell_chk), _ptr(self.cls_chk),
                                     _ptr(self.ws.origin), N, sim.L, _ptr(self.ws.pred),
                                     self.tol, _ptr(self.err), rej, _ptr(self.counters), go, sp),
@@ -513,6 +580,12 @@ class SurrogateRunner:
         """Blocks run, rejected (fallback) blocks and the largest per-cell
         check error so far (waits for the stream)."""
         c = self.counters.cpu().numpy()
-        return {"blocks": self.blocks, "fallback_blocks": int(c[1]),
-                "accepted_blocks": self.blocks - int(c[1]),
-                "max_check_error": float(c[:1].view(np.float32)[0])}
+        out = {"blocks": self.blocks, "fallback_blocks": int(c[1]),
+               "accepted_blocks": self.blocks - int(c[1]),
+               "max_check_error": float(c[:1].view(np.float32)[0]), "fallback": self.fallback}
+        if self.fallback == "cell":
+            cells = self.blocks * self.sim.N
+            out["fallback_cells"] = int(c[2])
+            out["accepted_cell_fraction"] = 1.0 - int(c[2]) / max(1, cells)
+            out["max_accumulated_error"] = float(self.acc.max().item())
+        return out

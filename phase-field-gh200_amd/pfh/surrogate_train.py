@@ -75,8 +75,8 @@ def _rel_l2_fields(a, b, origin) -> np.ndarray:
     return (torch.linalg.norm(fa - fb, dim=1) / torch.linalg.norm(fb, dim=1)).cpu().numpy()
 
 
-def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm: int = 300,
-             wl=None) -> dict:
+def evaluate(net: su.Net, blocks: int = 25, K: int | None = None, tol: float = 1e-3,
+             warm: int = 300, wl=None, **runner_kw) -> dict:
     wl = wl or params.workload_c2()
     plain = sim2d.Sim2D(wl)
     plain.capture_graph()
@@ -84,7 +84,8 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
     sur = sim2d.Sim2D(wl)
     sur.capture_graph()
     sur.run(warm)
-    runner = su.SurrogateRunner(sur, net, K=K, tol=tol)
+    runner = su.SurrogateRunner(sur, net, K=K, tol=tol, **runner_kw)
+    K = runner.K
     runner.prepare()
     # timing: plain K-step blocks from graphs, surrogate blocks
     torch.cuda.synchronize()
@@ -116,6 +117,11 @@ def evaluate(net: su.Net, blocks: int = 25, K: int = 8, tol: float = 1e-3, warm:
         "surrogate_wall_s": wall_sur,
         "accepted_blocks": st["accepted_blocks"],
         "fallback_blocks": st["fallback_blocks"],
+        "fallback": st["fallback"],
+        "fallback_cells": st.get("fallback_cells"),
+        "accepted_cell_fraction": st.get("accepted_cell_fraction"),
+        "max_accumulated_error": st.get("max_accumulated_error"),
+        "tol_inc": runner.tol_inc, "budget": runner.budget,
         "max_check_error": st["max_check_error"],
         "rel_l2_vs_plain_max": float(rel.max()),
         "rel_l2_vs_plain_mean": float(rel.mean()),

@@ -281,7 +281,7 @@ def test_accepted_block_commits_exactly():
     sim = sim2d.Sim2D(wl)
     sim.step(20)
     net = _random_net(3)
-    runner = su.SurrogateRunner(sim, net, K=8, tol=1e30)
+    runner = su.SurrogateRunner(sim, net, K=8, tol=1e30, fallback="population")
     say("one accepted block on C1, then rebuild S and moments")
     runner.block()
     st = runner.stats()
@@ -357,8 +357,8 @@ def test_graph_blocks_equal_eager_blocks():
         sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
         sa.step(10)
         sb.step(10)
-        ra = su.SurrogateRunner(sa, net, tol=tol, graphs=False)
-        rb = su.SurrogateRunner(sb, net, tol=tol, graphs=True)
+        ra = su.SurrogateRunner(sa, net, tol=tol, graphs=False, fallback="population")
+        rb = su.SurrogateRunner(sb, net, tol=tol, graphs=True, fallback="population")
         for b in range(4):
             say(f"tol {tol:g}: block {b} from step {sa.steps_done}")
             ra.block()
@@ -384,7 +384,7 @@ def test_rejected_block_equals_plain_steps():
     sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
     sa.step(12)
     sb.step(12)
-    r = su.SurrogateRunner(sa, net, tol=0.0, chunk=8)
+    r = su.SurrogateRunner(sa, net, tol=0.0, chunk=8, fallback="population")
     r.run(32)
     sb.step(32)
     torch.cuda.synchronize()
@@ -420,9 +420,10 @@ def test_abort_inside_chunk():
     sa, sb = sim2d.Sim2D(wl), sim2d.Sim2D(wl)
     sa.step(12)
     sb.step(12)
-    ra = _SwitchingRunner(sa, net, n_ok=2, chunk=4)
+    ra = _SwitchingRunner(sa, net, n_ok=2, chunk=4, fallback="population")
     ra.run(32)
-    rb = su.SurrogateRunner(sb, net, tol=1e30, graphs=False, chunk=1)
+    rb = su.SurrogateRunner(sb, net, tol=1e30, graphs=False, chunk=1,
+                            fallback="population")
     rb.run(16)
     sb.step(16)
     torch.cuda.synchronize()
@@ -519,3 +520,106 @@ def test_block_finish_equals_tumble_and_advance():
         f"(expected about {expected:.1f})")
     if expected > 5:
         assert (got["tumble_ctr"] != start["tumble_ctr"]).any()
+
+
+def test_cell_fallback_commits_the_check_field():
+    """Per-cell fallback with every cell rejected (tolerance 0): each cell
+    takes the check step's field, window position and class, S and the
+    moments are rebuilt from the committed fields exactly as the plain
+    path's own kernels rebuild them, and the block advances K steps."""
+    from pfh import abi
+    wl = params.workload_c1()
+    sim = sim2d.Sim2D(wl)
+    sim.step(20)
+    runner = su.SurrogateRunner(sim, _random_net(3), tol=0.0, graphs=False)
+    s0 = sim.steps_done
+    runner.block()
+    torch.cuda.synchronize()
+    st = runner.stats()
+    assert st["fallback_cells"] == sim.N and st["accepted_cell_fraction"] == 0.0
+    e = sim.steps_done
+    assert e == s0 + 8 and int(sim.stepc[e % 2].item()) == e
+    chk_cells = runner.cell_chk.cpu().numpy().view(abi.CELL_DTYPE)
+    cells = sim.cells()
+    assert np.array_equal(cells["gx0"], chk_cells["gx0"])
+    assert np.array_equal(cells["gy0"], chk_cells["gy0"])
+    assert np.array_equal(cells["cls"], chk_cells["cls"])
+    phi = sim.phi[e % 2].view(sim.N, -1)
+    chk = runner.phi_chk.view(sim.N, -1)
+    for n in range(sim.N):
+        wx, wy, tx0, ty0 = abi.CLASSES[int(cells["cls"][n])]
+        a = phi[n].view(abi.TILE_PITCH, -1)[ty0:ty0 + wy, tx0:tx0 + wx]
+        b = chk[n].view(abi.TILE_PITCH, -1)[ty0:ty0 + wy, tx0:tx0 + wx]
+        assert torch.equal(a, b), n
+    lib = abi.load()
+    S_ref = torch.zeros(sim.words, dtype=torch.int32, device=sim.dev)
+    flags = torch.zeros_like(sim.flags)
+    abi.check(lib.pfh2d_scatter_all(sim.phi[e % 2].data_ptr(), sim.cell.data_ptr(),
+                                    sim.cls.data_ptr(), S_ref.data_ptr(), sim.N, sim.L, sim.P,
+                                    flags.data_ptr(), None), "scatter_all")
+    cell_ref = sim.cell.clone()
+    abi.check(lib.pfh2d_init_moments(cell_ref.data_ptr(), sim.cls.data_ptr(),
+                                     sim.phi[e % 2].data_ptr(), sim.N, None), "init_moments")
+    torch.cuda.synchronize()
+    assert torch.equal(sim.S_slot(e % 3), S_ref)
+    assert torch.equal(sim.cell, cell_ref)
+    sim.step(2)
+    sim.sync()
+    assert not sim.fatal()
+
+
+def test_cell_larger_than_the_crop_is_rejected():
+    """The per-cell check counts the check field outside the 96-pixel crop:
+    with a prediction equal to the check field on the crop, cells of radius
+    20 (support about 66 pixels) pass at 1e-3, cells of radius 45 (support
+    about 116 pixels, wider than the crop) fail, since committing the crop
+    would cut their field off."""
+    from pfh import abi
+    lib = abi.load()
+    for R, want_rejected in ((20.0, 0), (45.0, 16)):
+        wl = params.monolayer(16, 768, R, 4, 4.0, 7, name=f"r{R:g}")
+        sim = sim2d.Sim2D(wl)
+        sim.step(40)
+        r = su.SurrogateRunner(sim, _random_net(3), tol=1e30, tol_inc=1e30, budget=1e30,
+                               graphs=False)
+        s = sim.steps_done
+        r.block()          # leaves the block's check field and crop origins behind
+        torch.cuda.synchronize()
+        pred = torch.zeros_like(r.ws.pred)
+        abi.check(lib.pfh_sur_recrop(r.phi_chk.data_ptr(), r.cell_chk.data_ptr(),
+                                     r.cls_chk.data_ptr(), r.ws.origin.data_ptr(), sim.N, sim.L,
+                                     pred.data_ptr(), None), "recrop")
+        acc = torch.zeros(sim.N, dtype=torch.float32, device=sim.dev)
+        err = torch.zeros_like(acc)
+        accept = torch.zeros(sim.N, dtype=torch.int32, device=sim.dev)
+        rej = torch.zeros(1, dtype=torch.int32, device=sim.dev)
+        pin = s % 2
+        abi.check(lib.pfh_sur_check_cells(
+            r.phi_chk.data_ptr(), r.cell_chk.data_ptr(), r.cls_chk.data_ptr(),
+            sim.phi[pin].data_ptr(), sim.cell.data_ptr(), sim.cls.data_ptr(),
+            r.ws.origin.data_ptr(), sim.N, sim.L, pred.data_ptr(), 1e-3, 1e30, 1e30,
+            acc.data_ptr(), err.data_ptr(), accept.data_ptr(), rej.data_ptr(), None, None),
+            "check cells")
+        torch.cuda.synchronize()
+        say(f"R={R:g}: rejected {int(rej.item())}, errors {err.min().item():.2e}.."
+            f"{err.max().item():.2e}")
+        assert int(rej.item()) == want_rejected
+        assert not sim.fatal()
+
+
+def test_long_run_stays_within_tolerance_of_plain():
+    """1000 steps (125 blocks of 8) of surrogate + per-cell fallback from
+    step 300 of the 2048^2 / 1024-cell monolayer (seed 7, not a training
+    seed): every cell's field stays within 1e-3 relative L2 of the plain
+    run, with the accumulated-error budget bounding the drift."""
+    from pfh import surrogate_train as st
+    net = su.Net.load(su.WEIGHTS_PATH)
+    r = st.evaluate(net, blocks=125)
+    say(f"1000 steps: rel L2 vs plain max {r['rel_l2_vs_plain_max']:.3e} mean "
+        f"{r['rel_l2_vs_plain_mean']:.3e}, accepted cells {r['accepted_cell_fraction']:.3f}, "
+        f"speedup {r['speedup']:.3f}")
+    assert not r["fatal"]
+    assert r["steps"] == 1000
+    assert r["rel_l2_vs_plain_max"] <= 1e-3
+    assert r["accepted_cell_fraction"] > 0.0
+

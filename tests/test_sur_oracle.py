@@ -93,3 +93,19 @@ def test_conv_impulse_fixes_tap_offsets_and_padding():
             if 0 <= py < H and 0 <= px < H:
                 expect[0, py, px, 3] = 0.75
             assert np.array_equal(out, expect), (y0, x0, t)
+
+
+def test_check_step_stability_rule():
+    """A rejected cell may take the check step's field only where one
+    explicit step of K dt is a stable integrator: on the monolayer's
+    coefficients the 8-step block qualifies (rate x K dt = 1.46) and the
+    16-step block does not (2.9), so the 16-step runner falls back to whole
+    populations. Halving dt doubles the admissible K."""
+    import dataclasses
+    from pfh import params, surrogate as su
+    p = params.workload_c2().params
+    assert su.check_step_stable(p, 8)
+    assert not su.check_step_stable(p, 16)
+    assert max(k for k in range(1, 40) if su.check_step_stable(p, k)) == 10
+    half = dataclasses.replace(p, dt=p.dt / 2)
+    assert max(k for k in range(1, 40) if su.check_step_stable(half, k)) == 21
