@@ -48,11 +48,36 @@ enum {
 #define PFH_TILE_AREA (288 * 288)
 #define PFH_NUM_CLASSES 6
 
-/* Runtime-only observer accumulator (common/velocity_moments.h:26-30). */
+/* Runtime-only observer accumulator (common/velocity_moments.h:26-30):
+ * q = integrated active (x, y), interaction (x, y) and stored-velocity
+ * rounding (x, y) velocity components; count = recorded steps. */
 typedef struct pfh_advection_accum {
     double q[6];
     unsigned long long count;
 } pfh_advection_accum;
+
+/* Velocity observation modes of pfh2d_launch_step (VelocityStep,
+ * include/kernels.cuh:216). */
+#define PFH_OBSERVE_DISABLED 0
+#define PFH_OBSERVE_ADVECTION 1
+#define PFH_OBSERVE_SPATIAL 2
+
+/* Per-cell velocity-moment accumulator (MomentAccum,
+ * common/velocity_moments.h:30-40), 240 bytes. Column pairs (x, y) of q:
+ * 0 active, 2 interaction, 4 interfacial, 6 overlap, 8 area, 10 advection
+ * correction. held: the interfacial and overlap rates (G, K) of the last
+ * spatial sample, then the normalised same-cell neighbour overlaps
+ * (sum phi phi_east / V, sum phi phi_north / V). */
+typedef struct pfh_moment_accum {
+    double q[12];
+    unsigned long long count;
+    double held[6];
+    unsigned long long spatial_count;
+    unsigned long long last_sample;
+    unsigned long long last_interval;
+    double last_totals[4];
+    double correction[4];
+} pfh_moment_accum;
 
 /* Per-cell state: include/kernels.cuh:27-57. 192 bytes, 64-byte aligned. */
 typedef struct pfh_cell_state {
@@ -110,7 +135,7 @@ typedef struct pfh2d_step_args {
     const unsigned long long* step_rd;
     unsigned long long* step_wr;
     uint32_t* flags;
-    void* moments;            /* observer accumulators: must be NULL */
+    pfh_moment_accum* moments; /* spatial observation: N accumulators */
     double physical_dt;
     int N;
     int L;
@@ -163,6 +188,8 @@ _Static_assert(offsetof(pfh_cell_state, reserved) == 120, "reserved moved");
 _Static_assert(offsetof(pfh_cell_state, M_pf) == 188, "M_pf moved");
 _Static_assert(sizeof(pfh_traj_packed_cell) == 48, "traj record size");
 _Static_assert(sizeof(pfh2d_replica) == 32, "replica record size");
+_Static_assert(sizeof(pfh_moment_accum) == 240, "MomentAccum is 240 B");
+_Static_assert(offsetof(pfh_moment_accum, held) == 104, "MomentAccum.held moved");
 #else
 static_assert(sizeof(pfh_cell_state) == 192, "pfh_cell_state must be 192 B");
 static_assert(alignof(pfh_cell_state) == 64, "pfh_cell_state alignment");
@@ -171,6 +198,8 @@ static_assert(offsetof(pfh_cell_state, reserved) == 120, "reserved moved");
 static_assert(offsetof(pfh_cell_state, M_pf) == 188, "M_pf moved");
 static_assert(sizeof(pfh_traj_packed_cell) == 48, "traj record size");
 static_assert(sizeof(pfh2d_replica) == 32, "replica record size");
+static_assert(sizeof(pfh_moment_accum) == 240, "MomentAccum is 240 B");
+static_assert(offsetof(pfh_moment_accum, held) == 104, "MomentAccum.held moved");
 #endif
 
 /* ---- library ---- */
@@ -200,14 +229,27 @@ int pfh2d_step_grid(int device);
 /* Replaces launch_step (include/kernels.cuh:223-225, src/kernels.cu:1329-1363).
  * One persistent work-queue launch advances every cell by one step.
  * l2_base/l2_bytes/l2_hit_ratio are accepted and ignored: gfx950 has no
- * L2 access-policy window. observation must be 0 (Disabled); the velocity
- * observers are outside this build (returns PFH_ERR_UNSUPPORTED). S_sc may
- * be NULL (no reference counterpart): the step then scatters nothing, for a
+ * L2 access-policy window. observation: PFH_OBSERVE_DISABLED (the plain
+ * step), PFH_OBSERVE_ADVECTION (each cell's advection accumulator records
+ * the step's velocity components, k_step_advection) or PFH_OBSERVE_SPATIAL
+ * (also the interfacial/overlap centroid rates into args->moments,
+ * k_step_moments, include/fused_moments.cuh:23-127). Observation changes no
+ * field, aggregate or cell state other than the accumulators. S_sc may be
+ * NULL (no reference counterpart): the step then scatters nothing, for a
  * caller that needs only the new field and cell states (the surrogate's check
  * step). */
 int pfh2d_launch_step(const pfh2d_step_args* args, int grid, void* stream,
                       const void* l2_base, size_t l2_bytes,
                       float l2_hit_ratio, int observation);
+/* Replaces launch_velocity_moments (include/velocity_moments.cuh:7-8,
+ * src/velocity_moments.cu:43-128): the independent dense observer. For each
+ * cell, from the step's input field, aggregate and cell state (the step it
+ * observes has not run yet): out[n].q[0..3] += physical_dt x (active x, y;
+ * interaction x, y velocity), q[4..9] += physical_dt x the interfacial,
+ * overlap and area centroid rates, q[10..11] += physical_dt x (advection
+ * rate - active - interaction), ++count. fp64 sums. */
+int pfh2d_velocity_moments(const pfh2d_step_args* args, pfh_moment_accum* out,
+                           double physical_dt, void* stream);
 /* pfh2d_launch_step gated on a device word: the launch does nothing unless
  * *run_if != 0 (run_if may be NULL: always runs). No reference counterpart:
  * lets a captured surrogate block carry steps whose execution depends on a

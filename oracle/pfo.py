@@ -37,6 +37,12 @@ CELL_DTYPE = np.dtype({
     "itemsize": 192,
 })
 
+# MomentAccum (common/velocity_moments.h:30-40): 240 bytes.
+MOMENT_DTYPE = np.dtype([("q", "<f8", (12,)), ("count", "<u8"), ("held", "<f8", (6,)),
+                         ("spatial_count", "<u8"), ("last_sample", "<u8"),
+                         ("last_interval", "<u8"), ("last_totals", "<f8", (4,)),
+                         ("correction", "<f8", (4,))])
+
 
 class StepArgs(C.Structure):
     _fields_ = [
@@ -52,6 +58,7 @@ class StepArgs(C.Structure):
         ("mot_d", C.c_double),
         ("polarity_seed", C.c_uint64), ("p_tumble", C.c_double),
         ("full_moment_every", C.c_int), ("clear_ahead_words", C.c_uint64),
+        ("observation", C.c_int), ("moments", C.c_void_p), ("physical_dt", C.c_double),
     ]
 
 
@@ -82,6 +89,7 @@ def load() -> C.CDLL:
         "pfo2d_verify_cells": (None, [_P, _P, _P, C.c_int, _P, _P]),
         "pfo2d_verify_S": (C.c_uint32, [_P, C.c_size_t]),
         "pfo2d_pack_traj": (None, [_P, _P, _P, C.c_int, C.c_int]),
+        "pfo2d_velocity_moments": (None, [_P, _P, C.c_double]),
         "pfo2d_morton_perm": (None, [_P, C.c_int, C.c_int, _P]),
         "pfo_philox": (None, [_P, _P, _P]),
         "pfo2d_tumble": (C.c_int, [C.c_uint64, C.c_int32, C.c_uint64, C.c_double, _P]),
@@ -153,6 +161,10 @@ class OracleSim2D:
         self.perm = np.arange(self.N, dtype=np.uint32)
         self.flags = np.zeros(8, np.uint32)
         self.vel = np.zeros(2 * self.N, np.float64)
+        # velocity observation of the next steps (0 none, 1 advection,
+        # 2 spatial into self.moments)
+        self.observation = 0
+        self.moments = np.zeros(self.N, MOMENT_DTYPE)
         self.steps_done = 0
         self._init(wl, init_phi)
 
@@ -223,6 +235,9 @@ class OracleSim2D:
         a.p_tumble = p.p_tumble()
         a.full_moment_every = p.full_moment_every
         a.clear_ahead_words = self.words
+        a.observation = self.observation
+        a.moments = _p(self.moments)
+        a.physical_dt = p.dt
         return a
 
     def step(self, n: int = 1) -> None:

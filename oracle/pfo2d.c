@@ -134,6 +134,89 @@ int pfo2d_tumble(uint64_t step, int32_t gid, uint64_t stream, double p_tumble,
     return 0;
 }
 
+/* ---- velocity observers (include/fused_moments.cuh:23-127,
+ * common/velocity_moments.h:42-100), in the kernels' order ---- */
+#define PFO_OBS_SUMS 6     /* gx, gy, kx, ky, bx, by partial arrays */
+
+/* One pixel of the spatial observer: weight 2 phi / V times the passive
+ * interfacial and overlap sources in fp64, times the offset from the input
+ * centroid; and the same-cell neighbour overlaps phi phi_east, phi phi_north. */
+static inline void pfo_obs_pixel(double* po, int t, int x, int y, double v, double e,
+                                 double w, double nn, double ss, double ne, double nw,
+                                 double se, double sw, double other, double gam,
+                                 double dwC, double repC, double g2iv, double cx,
+                                 double cy) {
+    const double axial = ((nn + ss) + e) + w;
+    const double lap = (((((4.0 * axial + ne) + nw) + se) + sw) - 20.0 * v) / 6.0;
+    const double weight = g2iv * v;
+    const double well = ((dwC * v) * (1.0 - v)) * (1.0 - 2.0 * v);
+    const double interfacial = weight * (gam * lap - well);
+    const double dx = (double)x - cx, dy = (double)y - cy;
+    po[0 * PFO_NT_MAX + t] += dx * interfacial;
+    po[1 * PFO_NT_MAX + t] += dy * interfacial;
+    const double overlap = (((-weight) * repC) * v) * other;
+    po[2 * PFO_NT_MAX + t] += dx * overlap;
+    po[3 * PFO_NT_MAX + t] += dy * overlap;
+    po[4 * PFO_NT_MAX + t] += v * e;
+    po[5 * PFO_NT_MAX + t] += v * nn;
+}
+
+static inline double pfo_linear_correction(double dt, uint64_t steps, double first,
+                                           double last) {
+    return steps ? dt * 0.5 * (double)(steps - 1ull) * (last - first) : 0.0;
+}
+
+/* The record after the velocity: record_advection (fused_moments.cuh:96-127)
+ * and, on a spatial step, begin_spatial_sample (velocity_moments.h:59-77) and
+ * the new held rates (fused_moments.cuh:55-88). sums = gx, gy, kx, ky, bx, by
+ * over the cell. vxf, vyf: the stored binary32 velocity. */
+static void pfo_obs_record(const pfo2d_step_args* A, int n, double ax, double ay,
+                           double sIx, double sIy, double vxf, double vyf,
+                           const double* sums) {
+    pfo_cell* cs = &A->cell[n];
+    const double ix = (double)A->mot_coeff * sIx;
+    const double iy = (double)A->mot_coeff * sIy;
+    const double dt = A->physical_dt;
+    pfo_moment* m = A->observation == 2 ? &A->moments[n] : NULL;
+    if (m) {
+        const double tot[4] = {cs->adv_q[0] + cs->adv_q[2], cs->adv_q[1] + cs->adv_q[3],
+                               cs->adv_q[4], cs->adv_q[5]};
+        const uint64_t interval = cs->adv_count - m->last_sample;
+        if (m->spatial_count) {
+            for (int j = 0; j < 4; ++j) m->q[4 + j] += (dt * (double)interval) * m->held[j];
+            for (int xy = 0; xy < 2; ++xy) {
+                const double beta = m->held[4 + xy];
+                m->q[10 + xy] += (beta - 1.0) * (tot[xy] - m->last_totals[xy]) +
+                                 beta * (tot[2 + xy] - m->last_totals[2 + xy]);
+            }
+        }
+        for (int j = 0; j < 4; ++j) m->last_totals[j] = tot[j];
+        m->last_interval = interval;
+        m->last_sample = cs->adv_count;
+        ++m->spatial_count;
+        m->held[4] = sums[4] / cs->V;
+        m->held[5] = sums[5] / cs->V;
+    }
+    cs->adv_q[0] += dt * ax;
+    cs->adv_q[1] += dt * ay;
+    cs->adv_q[2] += dt * ix;
+    cs->adv_q[3] += dt * iy;
+    cs->adv_q[4] += dt * ((vxf - ax) - ix);
+    cs->adv_q[5] += dt * ((vyf - ay) - iy);
+    ++cs->adv_count;
+    if (m) {
+        const double mrel = (double)cs->M_pf / 0.5;
+        for (int j = 0; j < 4; ++j) {
+            double rate = sums[j];
+            if (mrel != 1.0) rate *= mrel;
+            if (m->spatial_count > 1)
+                m->correction[j] +=
+                    pfo_linear_correction(dt, m->last_interval, m->held[j], rate);
+            m->held[j] = rate;
+        }
+    }
+}
+
 /* ---- binary32 and FP64 instantiations of the step ---- */
 #define REAL float
 #define SFX _f32
@@ -308,6 +391,79 @@ uint32_t pfo2d_verify_S(const uint32_t* S, size_t n) {
     uint32_t m = 0u;
     for (size_t i = 0; i < n; ++i) m = S[i] > m ? S[i] : m;
     return m;
+}
+
+/* The independent dense velocity observer (launch_velocity_moments,
+ * src/velocity_moments.cu:43-128) on one step's input: force, tumble-drawn
+ * polarity, then the centroid rates of the four velocity sources, every sum
+ * in pixel order with one fp64 accumulator per quantity. out[n] accumulates
+ * pdt x the rates as the GPU's pfh2d_velocity_moments does. */
+void pfo2d_velocity_moments(const pfo2d_step_args* A, pfo_moment* out, double pdt) {
+    for (int n = 0; n < A->N; ++n) {
+        const pfo_cell* c = &A->cell[n];
+        const pfo_shape sc = kCls[c->cls];
+        const float* tile = (const float*)A->phi_in + (size_t)n * PFO_TILE_AREA;
+#define AT(xx, yy)                                                                  \
+        (((unsigned)(xx) < (unsigned)sc.wx && (unsigned)(yy) < (unsigned)sc.wy)     \
+             ? tile[(size_t)(sc.ty0 + (yy)) * PFO_TILE_PITCH + sc.tx0 + (xx)] : 0.0f)
+        double f0 = 0.0, f1 = 0.0;
+        for (int y = 0; y < sc.wy; ++y)
+            for (int x = 0; x < sc.wx; ++x) {
+                const float p = AT(x, y);
+                const float gx = 0.5f * (AT(x + 1, y) - AT(x - 1, y));
+                const float gy = 0.5f * (AT(x, y + 1) - AT(x, y - 1));
+                const uint32_t all = A->S_rd[(size_t)pfo_wrapi(c->gy0 + y, A->L) * A->P +
+                                             pfo_wrapi(c->gx0 + x, A->L)];
+                const uint32_t self = pfo_q_of(p);
+                const float so = all >= self ? (float)(all - self) * PFO_QINV : NAN;
+                f0 += (double)(p * gx * so);
+                f1 += (double)(p * gy * so);
+            }
+        float theta = c->theta;
+        pfo2d_tumble(A->step, c->global_id, A->polarity_seed, A->p_tumble, &theta);
+        float sn, co;
+        pfo_sincosf_theta(theta, &sn, &co);
+        const double vel[4] = {(double)c->v_A * (double)co, (double)c->v_A * (double)sn,
+                               (double)A->mot_coeff * f0, (double)A->mot_coeff * f1};
+        const double cx = c->Cx / c->V, cy = c->Cy / c->V;
+        const double dw = (double)(A->bulk_scale * c->gamma);
+        const double vol = (double)(float)(A->vol_scale * (A->A0 - c->V));
+        const double mob = (double)(c->M_pf / 0.5f);
+        const double vx = (double)(float)(vel[0] + vel[2]);
+        const double vy = (double)(float)(vel[1] + vel[3]);
+        double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < sc.wy; ++y)
+            for (int x = 0; x < sc.wx; ++x) {
+                const double p = AT(x, y);
+                const double e = AT(x + 1, y), w = AT(x - 1, y);
+                const double nn = AT(x, y + 1), ss = AT(x, y - 1);
+                const double ne = AT(x + 1, y + 1), nw = AT(x - 1, y + 1);
+                const double se = AT(x + 1, y - 1), sw = AT(x - 1, y - 1);
+                const double lap =
+                    (((((4.0 * (((nn + ss) + e) + w) + ne) + nw) + se) + sw) - 20.0 * p) / 6.0;
+                const uint32_t all = A->S_rd[(size_t)pfo_wrapi(c->gy0 + y, A->L) * A->P +
+                                             pfo_wrapi(c->gx0 + x, A->L)];
+                const uint32_t self = pfo_q_of((float)p);
+                const double so = all >= self ? (double)((float)(all - self) * PFO_QINV) : NAN;
+                double src[4] = {(double)c->gamma * lap - ((dw * p) * (1.0 - p)) * (1.0 - 2.0 * p),
+                                 -((double)A->rep_coeff * p) * so, vol * p,
+                                 -(vx * (0.5 * (e - w)) + vy * (0.5 * (nn - ss)))};
+                if (mob != 1.0) { src[0] *= mob; src[1] *= mob; src[2] *= mob; }
+                const double wxp = ((2.0 * ((double)x - cx)) * p) / c->V;
+                const double wyp = ((2.0 * ((double)y - cy)) * p) / c->V;
+                for (int j = 0; j < 4; ++j) {
+                    m[2 * j] += wxp * src[j];
+                    m[2 * j + 1] += wyp * src[j];
+                }
+            }
+#undef AT
+        pfo_moment* z = &out[n];
+        for (int j = 0; j < 4; ++j) z->q[j] += pdt * vel[j];
+        for (int j = 0; j < 6; ++j) z->q[4 + j] += pdt * m[j];
+        z->q[10] += pdt * ((m[6] - vel[0]) - vel[2]);
+        z->q[11] += pdt * ((m[7] - vel[1]) - vel[3]);
+        ++z->count;
+    }
 }
 
 /* k_pack_traj, src/kernels.cu:1242-1265. */

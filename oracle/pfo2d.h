@@ -71,6 +71,16 @@ typedef struct __attribute__((aligned(64))) pfo_cell {
     float M_pf;
 } pfo_cell;
 
+/* 240-byte mirror of MomentAccum (common/velocity_moments.h:30-40). */
+typedef struct pfo_moment {
+    double q[12];
+    uint64_t count;
+    double held[6];
+    uint64_t spatial_count, last_sample, last_interval;
+    double last_totals[4];
+    double correction[4];
+} pfo_moment;
+
 typedef struct pfo_traj {
     int32_t global_id, cls;
     float cx, cy, vx, vy, theta, volume, perim, gamma, v_A, phi_max;
@@ -101,6 +111,11 @@ typedef struct pfo2d_step_args {
     double p_tumble;
     int full_moment_every;
     uint64_t clear_ahead_words;
+    /* velocity observation (VelocityStep, include/kernels.cuh:216): 0 none,
+     * 1 advection, 2 spatial (moments: N accumulators) */
+    int observation;
+    pfo_moment* moments;
+    double physical_dt;
 } pfo2d_step_args;
 
 /* One step over all cells (cells in perm order; each cell independent).
@@ -131,6 +146,7 @@ void pfo2d_verify_cells(const float* phi, const pfo_cell* cell,
                         float* out_outside_max);
 uint32_t pfo2d_verify_S(const uint32_t* S, size_t n);
 void pfo2d_morton_perm(const pfo_cell* cell, int N, int L, uint32_t* perm);
+void pfo2d_velocity_moments(const pfo2d_step_args* A, pfo_moment* out, double pdt);
 void pfo2d_pack_traj(const pfo_cell* cell, const uint8_t* cls, pfo_traj* out,
                      int N, int L);
 

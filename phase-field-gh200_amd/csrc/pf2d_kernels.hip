@@ -42,6 +42,9 @@ struct Ctl {
     int sx, sy, gx0, gy0, gx0n, gy0n, dcls, zero_tile, fm, pctr, tumbled;
     int dwx, dwy, dtx0, dty0, no_margin;
     float gam, dwC, volC, vA, cosp, sinp, theta, mrel, vx, vy;
+    // spatial velocity observer: 2 / V and the centroid of the step's input
+    // (include/fused_moments.cuh:16-21)
+    double og2iv, ocx, ocy;
 };
 
 // The cell a workgroup processes next: filled by the control wavefront
@@ -163,6 +166,9 @@ __device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_sta
     const float dwC = A.bulk_scale * gam;
     const float volC = (float)(A.vol_scale * (A.A0 - cs.V));
     const float mrel = cs.M_pf * (float)(1.0 / 0.5);
+    ctl->og2iv = 2.0 / cs.V;
+    ctl->ocx = cs.Cx / cs.V;
+    ctl->ocy = cs.Cy / cs.V;
 
     const int ex = cs.bb_hi_x - cs.bb_lo_x + 1;
     const int ey = cs.bb_hi_y - cs.bb_lo_y + 1;
@@ -249,18 +255,142 @@ __device__ void zero_outside_window(float* tile, int dwx, int dwy, int dtx0,
     }
 }
 
+// ---- velocity observers (include/fused_moments.cuh:23-127,
+// common/velocity_moments.h:42-100). Observation mode kObs: 0 none,
+// 1 advection, 2 spatial. Nothing here changes the step's physics. ----
+
+// Per-lane sums of the spatial observer: the interfacial (gx, gy) and
+// overlap (kx, ky) centroid rates and the same-cell neighbour overlaps
+// sum phi phi_east, sum phi phi_north (bx, by).
+struct ObsSums {
+    double gx = 0.0, gy = 0.0, kx = 0.0, ky = 0.0, bx = 0.0, by = 0.0;
+};
+
+// One pixel of the spatial observer (RelaxMoments::pixel,
+// include/fused_moments.cuh:36-53, and the beta sums of
+// src/kernels.cu:287-291): the pixel's weight 2 phi / V times its passive
+// interfacial and overlap sources, in fp64 from the binary32 stencil, times
+// its offset from the input centroid (g2iv = 2 / V, cx, cy).
+__device__ __forceinline__ void observe_pixel(ObsSums& o, int x, int y, float c, float e,
+                                              float w, float nn, float ss, float ne, float nw,
+                                              float se, float sw, float other, float gam,
+                                              float dwC, float repC, double g2iv, double cx,
+                                              double cy) {
+    const double v = (double)c;
+    const double axial = (((double)nn + (double)ss) + (double)e) + (double)w;
+    const double lap =
+        (((((4.0 * axial + (double)ne) + (double)nw) + (double)se) + (double)sw) - 20.0 * v) /
+        6.0;
+    const double weight = g2iv * v;
+    const double well = (((double)dwC * v) * (1.0 - v)) * (1.0 - 2.0 * v);
+    const double interfacial = weight * ((double)gam * lap - well);
+    const double dx = (double)x - cx, dy = (double)y - cy;
+    o.gx += dx * interfacial;
+    o.gy += dy * interfacial;
+    const double overlap = (((-weight) * (double)repC) * v) * (double)other;
+    o.kx += dx * overlap;
+    o.ky += dy * overlap;
+    o.bx += v * (double)e;
+    o.by += v * (double)nn;
+}
+
+// Discrete linear-interpolation correction of a held rate
+// (common/velocity_moments.h:80-85).
+__device__ __forceinline__ double linear_correction(double dt, unsigned long long steps,
+                                                    double first, double last) {
+    return steps ? dt * 0.5 * (double)(steps - 1ull) * (last - first) : 0.0;
+}
+
+// Thread 0 after the velocity: the advection record of the step
+// (record_advection, include/fused_moments.cuh:96-127) and, on a spatial
+// step, the sample's bookkeeping (begin_spatial_sample,
+// common/velocity_moments.h:59-77) and the new held rates
+// (RelaxMoments::finish, include/fused_moments.cuh:55-88). red holds the
+// wave sums: slots 2, 3 the neighbour overlaps, 4-7 gx, gy, kx, ky.
+template <int kObs>
+__device__ void record_observers(const pfh2d_step_args& A, int n, const Ctl& c, double sIx,
+                                 double sIy, const double* red) {
+    pfh_cell_state* cs = &A.cell[n];
+    pfh_advection_accum& a = cs->advection;
+    const double ax = (double)c.vA * (double)c.cosp;
+    const double ay = (double)c.vA * (double)c.sinp;
+    const double ix = (double)A.mot_coeff * sIx;
+    const double iy = (double)A.mot_coeff * sIy;
+    const double dt = A.physical_dt;
+    if constexpr (kObs == 2) {
+        pfh_moment_accum& m = A.moments[n];
+        const double tot[4] = {a.q[0] + a.q[2], a.q[1] + a.q[3], a.q[4], a.q[5]};
+        const unsigned long long interval = a.count - m.last_sample;
+        if (m.spatial_count) {
+            for (int j = 0; j < 4; ++j) m.q[4 + j] += (dt * (double)interval) * m.held[j];
+            for (int xy = 0; xy < 2; ++xy) {
+                const double beta = m.held[4 + xy];
+                m.q[10 + xy] += (beta - 1.0) * (tot[xy] - m.last_totals[xy]) +
+                                beta * (tot[2 + xy] - m.last_totals[2 + xy]);
+            }
+        }
+        for (int j = 0; j < 4; ++j) m.last_totals[j] = tot[j];
+        m.last_interval = interval;
+        m.last_sample = a.count;
+        ++m.spatial_count;
+        double bx = 0.0, by = 0.0;
+        for (int w = 0; w < kWaves; ++w) {
+            bx += red[w * 8 + 2];
+            by += red[w * 8 + 3];
+        }
+        m.held[4] = bx / cs->V;
+        m.held[5] = by / cs->V;
+    }
+    a.q[0] += dt * ax;
+    a.q[1] += dt * ay;
+    a.q[2] += dt * ix;
+    a.q[3] += dt * iy;
+    a.q[4] += dt * (((double)c.vx - ax) - ix);
+    a.q[5] += dt * (((double)c.vy - ay) - iy);
+    ++a.count;
+    if constexpr (kObs == 2) {
+        pfh_moment_accum& m = A.moments[n];
+        const double mrel = (double)cs->M_pf / 0.5;
+        for (int j = 0; j < 4; ++j) {
+            double rate = 0.0;
+            for (int w = 0; w < kWaves; ++w) rate += red[w * 8 + 4 + j];
+            // only the passive rates are held here; the area source has an
+            // exactly zero centroid moment
+            if (mrel != 1.0) rate *= mrel;
+            if (m.spatial_count > 1)
+                m.correction[j] += linear_correction(dt, m.last_interval, m.held[j], rate);
+            m.held[j] = rate;
+        }
+    }
+}
+
 // Velocity from the interaction integral (src/kernels.cu:300-342): wave
-// trees, then thread 0 adds the 16 wave sums in order and commits.
+// trees, then thread 0 adds the 16 wave sums in order and commits; with an
+// observer, the observer's sums ride in the same reduction.
+template <int kObs = 0>
 __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
-                                Ctl& c, double aIx, double aIy) {
+                                Ctl& c, double aIx, double aIy, ObsSums ob = ObsSums()) {
     const int tid = opaque_tid();
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
     aIx = wave_sum(aIx);
     aIy = wave_sum(aIy);
+    if constexpr (kObs == 2) {
+        ob.bx = wave_sum(ob.bx);
+        ob.by = wave_sum(ob.by);
+        ob.gx = wave_sum(ob.gx);
+        ob.gy = wave_sum(ob.gy);
+        ob.kx = wave_sum(ob.kx);
+        ob.ky = wave_sum(ob.ky);
+    }
     if (lane == 0) {
         sm.red[wave * 8 + 0] = aIx;
         sm.red[wave * 8 + 1] = aIy;
+        if constexpr (kObs == 2) {
+            double* r = sm.red + wave * 8;
+            r[2] = ob.bx; r[3] = ob.by;
+            r[4] = ob.gx; r[5] = ob.gy; r[6] = ob.kx; r[7] = ob.ky;
+        }
     }
     lds_barrier();
     if (threadIdx.x == 0) {
@@ -280,6 +410,7 @@ __device__ void reduce_velocity(const pfh2d_step_args& A, int n, Smem& sm,
         cs->vy = (float)vyd;
         cs->Ix = sIx;
         cs->Iy = sIy;
+        if constexpr (kObs != 0) record_observers<kObs>(A, n, c, sIx, sIy, sm.red);
     }
     lds_barrier();
 }
@@ -601,7 +732,7 @@ __device__ __forceinline__ Dom cell_dom(const pfh2d_step_args& A, const Slot& c)
     }
 }
 
-template <int CLS, bool kCheck = false, bool kBatch = false>
+template <int CLS, bool kCheck = false, bool kBatch = false, int kObs = 0>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt,
                             int& pre, int& pslot) {
@@ -756,6 +887,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     // ---- phase A: interaction integral and the held passive RHS ----
     float pas[RB];
     double aIx = 0.0, aIy = 0.0;
+    ObsSums ob;
     if (active) {
         uint32_t neg1 = 0u;
         Roll q;
@@ -768,6 +900,9 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 load_row(cbw + (k + 2) * PX, q.nW, q.nC, q.nE);
                 const float head = passive_head(q, gam, dwC, volC);
                 const float So = s_other_lane(qS[k], q.cC, neg1);
+                if constexpr (kObs == 2)
+                    observe_pixel(ob, x, y0 + k, q.cC, q.cE, q.cW, q.nC, q.sC, q.nE, q.nW,
+                                  q.sE, q.sW, So, gam, dwC, repC, c.og2iv, c.ocx, c.ocy);
                 if (wave_any(So != 0.0f)) {
                     pas[k] = head - repC * q.cC * So;
                     const float gxv = 0.5f * (q.cE - q.cW);
@@ -790,7 +925,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     }
 
     if (control) fetch4a(A, sm, nf, nxt);       // visible after the next barrier
-    reduce_velocity(A, n, sm, c, aIx, aIy);
+    reduce_velocity<kObs>(A, n, sm, c, aIx, aIy, ob);
     // A class-0 cell next in the queue gets its window copied in ahead:
     // piece 1 into the other slot (its last reader was the previous cell) as
     // soon as the first waves finish phase B, piece 2 once every wave has.
@@ -811,7 +946,10 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     bool nonfin = false, clampf = false, ovf = false;
     auto update = [&](float cC, float gxv, float gyv, float pas_k) -> float {
         const float adv = vx * gxv + vy * gyv;
-        const float rhs = (mrel == 1.0f) ? pas_k - adv : mrel * pas_k - adv;
+        // the reference takes passive - adv when mrel == 1 and mrel * passive
+        // - adv otherwise (include/kernels.cuh:118-138); 1.0f * x is x exactly,
+        // so one expression gives both without a per-pixel select
+        const float rhs = mrel * pas_k - adv;
         return cC + dt * rhs;
     };
     const int a = x - sx;
@@ -1003,7 +1141,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
 // Global-memory path: class 4 (window zeros synthesized) and the fallback
 // class 5 (the tile's own one-pixel zero ring), src/kernels.cu:604-937.
 // -------------------------------------------------------------------------
-template <int CLS, bool kBatch = false>
+template <int CLS, bool kBatch = false, int kObs = 0>
 __device__ void cell_global(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt) {
     constexpr int WX = kShapes[CLS].wx, WY = kShapes[CLS].wy;
@@ -1027,6 +1165,7 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
 
     double aIx = 0.0, aIy = 0.0;
     uint32_t negc = 0u;
+    ObsSums ob;
     for (int p = tid; p < WX * WY; p += kNT) {
         const int y = p / WX, x = p - (p / WX) * WX;
         const float cc = pin(x, y);
@@ -1039,8 +1178,13 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         else negc += 1u;
         aIx += (double)(cc * gxv * So);
         aIy += (double)(cc * gyv * So);
+        if constexpr (kObs == 2)
+            observe_pixel(ob, x, y, cc, pin(x + 1, y), pin(x - 1, y), pin(x, y + 1),
+                          pin(x, y - 1), pin(x + 1, y + 1), pin(x - 1, y + 1),
+                          pin(x + 1, y - 1), pin(x - 1, y - 1), So, c.gam, c.dwC,
+                          A.rep_coeff, c.og2iv, c.ocx, c.ocy);
     }
-    reduce_velocity(A, n, sm, c, aIx, aIy);
+    reduce_velocity<kObs>(A, n, sm, c, aIx, aIy, ob);
 
     const float gam = c.gam, dwC = c.dwC, volC = c.volC, repC = A.rep_coeff;
     const float vx = c.vx, vy = c.vy, mrel = c.mrel, dt = A.dt;
@@ -1128,16 +1272,16 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
 // Classes other than the round class 0 run out of line, with their own
 // register allocation, so their longer per-thread columns (up to 29 rows) do
 // not raise the pressure of the hot class-0 path.
-template <int CLS, bool kBatch = false>
+template <int CLS, bool kBatch = false, int kObs = 0>
 __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
                                        unsigned long long step, Smem* sm, int cur) {
     if constexpr (CLS <= kClassStagedMax) {
         int none = 0, slot0 = 0;
-        cell_staged<CLS, false, kBatch>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1, none,
-                                        slot0);
+        cell_staged<CLS, false, kBatch, kObs>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1,
+                                              none, slot0);
     }
     else
-        cell_global<CLS, kBatch>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
+        cell_global<CLS, kBatch, kObs>(*A, n, step, *sm, sm->slot[cur].ctl, cur ^ 1);
 }
 
 // -------------------------------------------------------------------------
@@ -1148,7 +1292,7 @@ __device__ __noinline__ void cell_cold(const pfh2d_step_args* A, int n,
 // cells; each S slot one arena holding every replica's aggregate; the
 // clear-ahead sweeps the whole arena), and each cell's side, S pitch, S
 // offset and polarity stream come from its replica's record.
-template <bool kCheck, bool kBatch = false>
+template <bool kCheck, bool kBatch = false, int kObs = 0>
 __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const int* run_if,
                                                  const pfh2d_replica* reps,
                                                  const uint16_t* cell_rep) {
@@ -1198,14 +1342,14 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         if (cls < 0) break;
         switch (cls) {
             case 0:
-                cell_staged<0, kCheck, kBatch>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1, pre,
-                                               pslot);
+                cell_staged<0, kCheck, kBatch, kObs>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1,
+                                                     pre, pslot);
                 break;
-            case 1: cell_cold<1, kBatch>(&sm.args, n, step, &sm, cur); break;
-            case 2: cell_cold<2, kBatch>(&sm.args, n, step, &sm, cur); break;
-            case 3: cell_cold<3, kBatch>(&sm.args, n, step, &sm, cur); break;
-            case 4: cell_cold<4, kBatch>(&sm.args, n, step, &sm, cur); break;
-            case 5: cell_cold<5, kBatch>(&sm.args, n, step, &sm, cur); break;
+            case 1: cell_cold<1, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
+            case 2: cell_cold<2, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
+            case 3: cell_cold<3, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
+            case 4: cell_cold<4, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
+            case 5: cell_cold<5, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
             default:                       // unsupported id, flagged at fetch
                 if (tid < kControlLanes) fetch_all<kBatch>(A, sm, cur ^ 1, step);
                 lds_barrier();
@@ -1597,6 +1741,134 @@ __global__ __launch_bounds__(kNT) void k_morton_sort(const pfh_cell_state* cell,
     for (int i = tid; i < N; i += kNT) perm[i] = idx[cur][i];
 }
 
+// -------------------------------------------------------------------------
+// Independent dense velocity observer (src/velocity_moments.cu:43-128): one
+// 256-thread workgroup per cell, straight from the step's input field and
+// aggregate in global memory (no window staging, no shift, no class
+// hysteresis: none of the step's code paths), two passes over the window:
+// the interaction force, then the centroid rates of the four velocity
+// sources (passive interfacial, overlap, area; advection) in fp64.
+// Lane-strided sums, 64-lane trees, waves in ascending order.
+// -------------------------------------------------------------------------
+constexpr int kObsNT = 256, kObsWaves = kObsNT / kWave;
+
+template <int K>
+__device__ void obs_block_sum(double (&v)[K], double* sh) {
+    const int lane = (int)threadIdx.x & (kWave - 1), wave = (int)threadIdx.x / kWave;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        v[j] = wave_sum(v[j]);
+        if (lane == 0) sh[j * kObsWaves + wave] = v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double t = 0.0;
+            for (int w = 0; w < kObsWaves; ++w) t += sh[j * kObsWaves + w];
+            sh[j * kObsWaves] = t;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kObsNT) void k_velocity_observe(const pfh2d_step_args A,
+                                                             pfh_moment_accum* out,
+                                                             double pdt) {
+    __shared__ double sh[8 * kObsWaves];
+    __shared__ double vel[4];
+    const int n = (int)blockIdx.x;
+    const pfh_cell_state& c = A.cell[n];
+    const Shape sc = shape_of((int)c.cls);
+    const int L = A.L, P = A.P, gx0 = c.gx0, gy0 = c.gy0;
+    const float* tile = A.phi_in + (size_t)n * kTileArea;
+    auto at = [&](int x, int y) -> float {
+        return ((unsigned)x < (unsigned)sc.wx && (unsigned)y < (unsigned)sc.wy)
+            ? tile[(size_t)(sc.ty0 + y) * kTilePitch + sc.tx0 + x] : 0.0f;
+    };
+    // S_other at a window pixel; an invalid aggregate (S below the cell's own
+    // share) gives NaN, so the sample is refused downstream
+    auto other = [&](int x, int y, float p) -> float {
+        const uint32_t all = A.S_rd[(size_t)wrap_once(gy0 + y, L) * P + wrap_once(gx0 + x, L)];
+        const uint32_t self = q_of(p);
+        return all >= self ? (float)(all - self) * kQInv : __int_as_float(0x7fc00000);
+    };
+    const int W = sc.wx * sc.wy;
+    double f[2] = {0.0, 0.0};
+    for (int k = (int)threadIdx.x; k < W; k += kObsNT) {
+        const int y = k / sc.wx, x = k - y * sc.wx;
+        const float p = at(x, y);
+        const float gx = 0.5f * (at(x + 1, y) - at(x - 1, y));
+        const float gy = 0.5f * (at(x, y + 1) - at(x, y - 1));
+        const float so = other(x, y, p);
+        // the same binary32 product as the step's interaction integrand
+        f[0] += (double)(p * gx * so);
+        f[1] += (double)(p * gy * so);
+    }
+    obs_block_sum(f, sh);
+    if (threadIdx.x == 0) {
+        const unsigned long long step = *A.step_rd;
+        const Philox4 r = philox4x32_10(
+            (uint32_t)(step & 0xFFFFFFFFull), (uint32_t)(step >> 32), (uint32_t)c.global_id,
+            0xA5A5A5A5u, (uint32_t)(A.polarity_seed & 0xFFFFFFFFull),
+            (uint32_t)(A.polarity_seed >> 32));
+        float theta = c.theta;
+        if (uniform53(r.v[0], r.v[1]) < A.p_tumble)
+            theta = (float)(2.0 * kPi * uniform53(r.v[2], r.v[3]));
+        double sd, cd;
+        sincos_d((double)theta, &sd, &cd);
+        vel[0] = (double)c.v_A * (double)(float)cd;
+        vel[1] = (double)c.v_A * (double)(float)sd;
+        vel[2] = (double)A.mot_coeff * sh[0];
+        vel[3] = (double)A.mot_coeff * sh[kObsWaves];
+    }
+    __syncthreads();
+    const double cx = c.Cx / c.V, cy = c.Cy / c.V;
+    const double dw = (double)(A.bulk_scale * c.gamma);
+    const double vol = (double)(float)(A.vol_scale * (A.A0 - c.V));
+    const double mob = (double)(c.M_pf / 0.5f);
+    // the advection uses the stored (binary32) velocity
+    const double vx = (double)(float)(vel[0] + vel[2]);
+    const double vy = (double)(float)(vel[1] + vel[3]);
+    double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = (int)threadIdx.x; k < W; k += kObsNT) {
+        const int y = k / sc.wx, x = k - y * sc.wx;
+        const double p = (double)at(x, y);
+        const double e = at(x + 1, y), w = at(x - 1, y);
+        const double nn = at(x, y + 1), ss = at(x, y - 1);
+        const double ne = at(x + 1, y + 1), nw = at(x - 1, y + 1);
+        const double se = at(x + 1, y - 1), sw = at(x - 1, y - 1);
+        const double lap =
+            (((((4.0 * (((nn + ss) + e) + w) + ne) + nw) + se) + sw) - 20.0 * p) / 6.0;
+        const double so = (double)other(x, y, (float)p);
+        double src[4] = {(double)c.gamma * lap - ((dw * p) * (1.0 - p)) * (1.0 - 2.0 * p),
+                         -((double)A.rep_coeff * p) * so, vol * p,
+                         -(vx * (0.5 * (e - w)) + vy * (0.5 * (nn - ss)))};
+        // the mobility scales the passive sources, never the advection
+        if (mob != 1.0) {
+            src[0] *= mob;
+            src[1] *= mob;
+            src[2] *= mob;
+        }
+        const double wxp = ((2.0 * ((double)x - cx)) * p) / c.V;
+        const double wyp = ((2.0 * ((double)y - cy)) * p) / c.V;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            m[2 * j] += wxp * src[j];
+            m[2 * j + 1] += wyp * src[j];
+        }
+    }
+    obs_block_sum(m, sh);
+    if (threadIdx.x == 0) {
+        pfh_moment_accum& z = out[n];
+        for (int j = 0; j < 4; ++j) z.q[j] += pdt * vel[j];
+        for (int j = 0; j < 6; ++j) z.q[4 + j] += pdt * sh[j * kObsWaves];
+        z.q[10] += pdt * ((sh[6 * kObsWaves] - vel[0]) - vel[2]);
+        z.q[11] += pdt * ((sh[7 * kObsWaves] - vel[1]) - vel[3]);
+        ++z.count;
+    }
+}
+
 }  // namespace pfh
 
 // -------------------------------------------------------------------------
@@ -1656,9 +1928,10 @@ int pfh2d_step_grid(int device) {
 
 namespace {
 int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* stream,
-                bool check = false) {
+                bool check = false, int observation = PFH_OBSERVE_DISABLED) {
     if (!a) return invalid("args");
-    if (a->moments) return invalid("moments must be NULL");
+    if (observation == PFH_OBSERVE_SPATIAL && !a->moments)
+        return invalid("a spatial observation needs the moment accumulators (moments)");
     if (grid <= 0 || a->N < 0 || a->L <= 0 || a->P < a->L || !a->phi_in ||
         !a->phi_out || !a->S_rd || !a->cell || !a->cell_cls ||
         !a->perm || !a->cursor_use || !a->cursor_clear || !a->step_rd ||
@@ -1673,12 +1946,19 @@ int step_launch(const pfh2d_step_args* a, const int* run_if, int grid, void* str
         return invalid("aggregate larger than 4 GiB");
     const pfh2d_replica* no_reps = nullptr;
     const uint16_t* no_map = nullptr;
+    const dim3 g((unsigned)grid), b(pfh::kNT);
     if (check)
-        hipLaunchKernelGGL((pfh::k_step<true, false>), dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                           as_stream(stream), *a, run_if, no_reps, no_map);
+        hipLaunchKernelGGL((pfh::k_step<true, false>), g, b, 0, as_stream(stream), *a, run_if,
+                           no_reps, no_map);
+    else if (observation == PFH_OBSERVE_ADVECTION)
+        hipLaunchKernelGGL((pfh::k_step<false, false, 1>), g, b, 0, as_stream(stream), *a,
+                           run_if, no_reps, no_map);
+    else if (observation == PFH_OBSERVE_SPATIAL)
+        hipLaunchKernelGGL((pfh::k_step<false, false, 2>), g, b, 0, as_stream(stream), *a,
+                           run_if, no_reps, no_map);
     else
-        hipLaunchKernelGGL((pfh::k_step<false, false>), dim3((unsigned)grid), dim3(pfh::kNT), 0,
-                           as_stream(stream), *a, run_if, no_reps, no_map);
+        hipLaunchKernelGGL((pfh::k_step<false, false>), g, b, 0, as_stream(stream), *a, run_if,
+                           no_reps, no_map);
     return launch_status("k_step");
 }
 }  // namespace
@@ -1688,9 +1968,20 @@ int pfh2d_launch_step(const pfh2d_step_args* a, int grid, void* stream,
                       int observation) {
     (void)l2_base; (void)l2_bytes; (void)l2_hit_ratio;
     if (!a) return invalid("args");
-    if (observation != 0)
-        return unsupported("velocity observation is not part of this build");
-    return step_launch(a, nullptr, grid, stream);
+    if (observation < PFH_OBSERVE_DISABLED || observation > PFH_OBSERVE_SPATIAL)
+        return invalid("observation is 0 (disabled), 1 (advection) or 2 (spatial)");
+    return step_launch(a, nullptr, grid, stream, false, observation);
+}
+
+int pfh2d_velocity_moments(const pfh2d_step_args* a, pfh_moment_accum* out,
+                           double physical_dt, void* stream) {
+    if (!a || !out) return invalid("velocity_moments: args and out are required");
+    if (a->N <= 0 || a->L <= 286 || a->P < a->L || !a->phi_in || !a->S_rd || !a->cell ||
+        !a->step_rd)
+        return invalid("velocity_moments: step args");
+    hipLaunchKernelGGL(pfh::k_velocity_observe, dim3((unsigned)a->N), dim3(pfh::kObsNT), 0,
+                       as_stream(stream), *a, out, physical_dt);
+    return launch_status("k_velocity_observe");
 }
 
 int pfh2d_launch_step_if(const pfh2d_step_args* a, const int* run_if, int grid,
@@ -1708,7 +1999,6 @@ int pfh2d_launch_batch_step(const pfh2d_step_args* a, const pfh2d_replica* repli
                             const uint16_t* cell_replica, int num_replicas, int max_side,
                             int max_pitch, int grid, void* stream) {
     if (!a || !replicas || !cell_replica) return invalid("batch step: null argument");
-    if (a->moments) return invalid("moments must be NULL");
     if (num_replicas <= 0 || num_replicas > 65536 || grid <= 0 || a->N <= 0 ||
         !a->phi_in || !a->phi_out || !a->S_rd || !a->S_sc || !a->cell || !a->cell_cls ||
         !a->perm || !a->cursor_use || !a->cursor_clear || !a->step_rd || !a->step_wr ||

@@ -103,6 +103,82 @@ struct SimOverrides {
     void apply(SimParams& p, const SimParams& cli) const;
 };
 
+// Velocity-component recording (--velocity-moments and friends).
+struct VelocityOptions {
+    std::string path;
+    int spatial_stride = 100;
+    long long dense_start = 10000;
+    bool reference = false;
+};
+
+// Which steps measure the spatial moments (include/velocity_sampling.hpp):
+// every step a multiple of the stride, and every step of the dense start
+// interval and the first step after (re)start, which graphs cannot capture.
+// Graphs cover a period of lcm(6, stride) steps (phi parity, S rotation and
+// the stride), split into pieces of `stride` steps for long strides.
+class VelocitySampling {
+public:
+    void configure(bool enabled, bool reference, int stride);
+    void start_run(long long step, long long dense_steps) { dense_until_ = step + dense_steps; }
+    int graph_period() const { return period_; }
+    int graph_steps() const { return graph_steps_; }
+    int graph_count() const { return period_ / graph_steps_; }
+    int graph_index(long long step) const { return (int)(step % period_) / graph_steps_; }
+    bool can_replay(long long step) const {
+        return !enabled_ || reference_ || (primed_ && step >= dense_until_);
+    }
+    bool spatial_sample(int slot, long long step, bool capture) const {
+        return slot % stride_ == 0 || (!capture && (!primed_ || step < dense_until_));
+    }
+    void advanced() { primed_ = true; }
+
+private:
+    bool enabled_ = false, reference_ = false, primed_ = false;
+    int stride_ = 1, period_ = kGraphBody, graph_steps_ = kGraphBody;
+    long long dense_until_ = 0;
+};
+
+// PFVMOM4 velocity-moment output (include/velocity_output.cuh,
+// common/velocity_format.h): a 56-byte header, then frames of an int64 step
+// and cells x 26 float64 columns (id, centroid x/y, theta, gamma, M_pf, v_A,
+// V, recorded steps, the 12 cumulative integrals, spatial samples, and the
+// four instantaneous endpoint rates Gx, Gy, Kx, Ky).
+class VelocityOutput {
+public:
+    VelocityOutput() = default;
+    ~VelocityOutput();
+    VelocityOutput(const VelocityOutput&) = delete;
+    VelocityOutput& operator=(const VelocityOutput&) = delete;
+
+    bool configure(const VelocityOptions& options, int cells, double dt);
+    bool open(const pfh2d_step_args& state, long long step, void* stream);
+    bool capture(const pfh2d_step_args& state, long long step, void* stream);
+    bool close();
+    bool enabled() const { return moments_ != nullptr; }
+    bool reference() const { return options_.reference; }
+    pfh_moment_accum* accumulators() const { return moments_; }
+    long long last_frame() const { return last_frame_; }
+    VelocitySampling& sampling() { return sampling_; }
+    const VelocitySampling& sampling() const { return sampling_; }
+
+private:
+    bool readback(const pfh2d_step_args& state, void* stream);
+    bool pack_frame(long long step, int side);
+    bool write(const void* data, size_t bytes);
+
+    VelocityOptions options_;
+    VelocitySampling sampling_;
+    double dt_ = 0.0;
+    pfh_moment_accum* moments_ = nullptr;      // device, N
+    pfh_moment_accum* endpoint_ = nullptr;     // device, N
+    std::vector<pfh_cell_state> cells_;
+    std::vector<pfh_moment_accum> integrals_, endpoints_;
+    std::vector<double> rows_;
+    std::FILE* file_ = nullptr;
+    long long start_step_ = 0, last_frame_ = -1;
+    bool failed_ = false;
+};
+
 struct RunOptions {
     std::string out_path;
     std::string initial_centres_path;
@@ -116,6 +192,7 @@ struct RunOptions {
     bool use_graph = true;
     bool morton = false;
     bool strict = false;
+    VelocityOptions velocity;
 };
 
 // A checkpoint's state: cell records as device CellState images (with the
@@ -176,8 +253,11 @@ private:
     bool configure_and_capture();
     bool seed_positions(std::vector<float>& cx, std::vector<float>& cy);
     pfh2d_step_args args_for_slot(int slot) const;
-    bool launch_one(int slot);
+    bool launch_one(int slot, bool capturing = false);
     bool build_graph();
+    bool replay_graph(bool* replayed, long long next_stop);
+    bool open_observations();
+    bool close_observations(bool valid_state);
     void print_line();
     bool fatal_flag_set();
     void report_flags();
@@ -207,8 +287,10 @@ private:
     uint32_t* d_smax_ = nullptr;
     pfh_traj_packed_cell* h_traj_ = nullptr;   // mapped pinned host memory
     pfh_traj_packed_cell* d_traj_ = nullptr;
-    void* graph_ = nullptr;        // hipGraph_t of one six-step body
-    void* graph_exec_ = nullptr;   // hipGraphExec_t
+    // hipGraph_t / hipGraphExec_t of each graph piece: one six-step body, or
+    // the pieces of the observation period
+    std::vector<void*> graphs_, graph_execs_;
+    VelocityOutput velocity_;
     std::FILE* traj_fp_ = nullptr;
     long long traj_frames_ = 0;
 };

@@ -53,6 +53,10 @@ static void usage(const char* argv0) {
         "  --out <path>           plain-text trajectory output\n"
         "  --trajectory-samples <int>  evenly spaced trajectory frames (100)\n"
         "  --trajectory-interval <int> steps between frames (overrides samples)\n"
+        "  --velocity-moments <path>  new PFVMOM4 velocity-component output\n"
+        "  --velocity-spatial-stride <int>  spatial measurement stride, 1..1000 (100)\n"
+        "  --velocity-dense-start <int>  initial densely measured steps (10000)\n"
+        "  --velocity-moments-reference  independent dense observer for validation\n"
         "  --self-test            run coefficient, RNG, and force-sign checks\n"
         "  -h, --help             this message\n\n"
         "checkpointing (CELL schema 1)\n"
@@ -208,6 +212,7 @@ int main(int argc, char** argv) {
     SimOverrides ov;
     long long device = 0, iv = 0;
     bool self_test = false, no_final = false, samples_given = false, interval_given = false;
+    bool velocity_given = false;
     std::string ckpt_in, ckpt_dir_flag;
     for (int i = 1; i < argc; ++i) {
         const char* a = argv[i];
@@ -332,19 +337,37 @@ int main(int argc, char** argv) {
             opt.out_path = need();
         } else if (is("--self-test")) {
             self_test = true;
+        } else if (is("--velocity-moments")) {
+            opt.velocity.path = need();
+            velocity_given = true;
+        } else if (is("--velocity-spatial-stride")) {
+            if (!parse_i(a, need(), &iv, 1, 1000)) return 2;
+            opt.velocity.spatial_stride = (int)iv;
+            velocity_given = true;
+        } else if (is("--velocity-dense-start")) {
+            if (!parse_i(a, need(), &iv, 0, 1000000000)) return 2;
+            opt.velocity.dense_start = iv;
+            velocity_given = true;
+        } else if (is("--velocity-moments-reference")) {
+            opt.velocity.reference = true;
+            velocity_given = true;
         } else if (is("--phase-field-mobility-map") || is("--boundary-out") ||
-                   is("--boundary-interval") || is("--boundary-compression") ||
-                   is("--velocity-moments") || is("--velocity-spatial-stride") ||
-                   is("--velocity-dense-start") || is("--velocity-moments-reference")) {
-            std::fprintf(stderr, "[fatal] %s is not supported by this build (boundary, velocity "
-                                 "observer and mobility-map outputs are outside it)\n", a);
+                   is("--boundary-interval") || is("--boundary-compression")) {
+            std::fprintf(stderr, "[fatal] %s is not supported by this build (boundary and "
+                                 "mobility-map outputs are outside it)\n", a);
             return 2;
         } else {
             std::fprintf(stderr, "[fatal] unknown option '%s' (try --help)\n", a);
             return 2;
         }
     }
-    if (!distinct_output_paths({opt.out_path})) return 2;
+    if (velocity_given &&
+        (opt.velocity.path.empty() || (opt.out_path.empty() && opt.bench_steps == 0))) {
+        std::fprintf(stderr, "[fatal] velocity recording requires --velocity-moments and --out "
+                             "(--bench measures the observer without writing files)\n");
+        return 2;
+    }
+    if (!distinct_output_paths({opt.out_path, opt.velocity.path})) return 2;
     CheckpointData ckpt;
     if (!ckpt_in.empty()) {
         if (!opt.initial_centres_path.empty()) {
@@ -385,7 +408,8 @@ int main(int argc, char** argv) {
             opt.ckpt_dir = src.parent_path().empty() ? std::string(".")
                                                      : src.parent_path().string();
         }
-        if (!distinct_output_paths({opt.out_path, opt.ckpt_dir + "/checkpoint.bin",
+        if (!distinct_output_paths({opt.out_path, opt.velocity.path,
+                                    opt.ckpt_dir + "/checkpoint.bin",
                                     opt.ckpt_dir + "/checkpoint_failed.bin"}))
             return 2;
         std::error_code ec;

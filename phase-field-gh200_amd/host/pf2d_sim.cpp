@@ -54,8 +54,9 @@ void Sim::request_termination(int) { g_terminate = 1; }
 
 Sim::~Sim() {
     if (traj_fp_) std::fclose(traj_fp_);
-    if (graph_exec_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
-    if (graph_) (void)hipGraphDestroy((hipGraph_t)graph_);
+    (void)velocity_.close();
+    for (void* x : graph_execs_) (void)hipGraphExecDestroy((hipGraphExec_t)x);
+    for (void* g : graphs_) (void)hipGraphDestroy((hipGraph_t)g);
     if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
     for (void* p : {(void*)d_phi_[0], (void*)d_phi_[1], (void*)d_S_, (void*)d_cell_,
                     (void*)d_cls_, (void*)d_perm_, (void*)d_cursor_, (void*)d_step_,
@@ -141,6 +142,7 @@ bool Sim::alloc_device(const SimParams& p, const RunOptions& opt, int device) {
 
 bool Sim::configure_and_capture() {
     const long long total = p_.total_steps();
+    if (!velocity_.configure(opt_.velocity, p_.num_cells, p_.dt)) return false;
     trajectory_every_ = opt_.traj_interval > 0
         ? opt_.traj_interval
         : std::max<long long>(1, total / std::max(1, opt_.traj_samples));
@@ -279,7 +281,7 @@ pfh2d_step_args Sim::args_for_slot(int slot) const {
     a.step_rd = d_step_ + pin;
     a.step_wr = d_step_ + pout;
     a.flags = d_flags_;
-    a.moments = nullptr;
+    a.moments = velocity_.reference() ? nullptr : velocity_.accumulators();
     a.physical_dt = p_.dt;
     a.N = p_.num_cells;
     a.L = side_;
@@ -299,38 +301,89 @@ pfh2d_step_args Sim::args_for_slot(int slot) const {
     return a;
 }
 
-bool Sim::launch_one(int slot) {
+// slot: the step's index in the graph period (phi parity = slot % 2, S
+// rotation = slot % 3, spatial sample when slot % stride == 0). With the
+// dense reference observer, its measurement of the step's input precedes
+// the plain step; otherwise the step itself records (src/sim.cu:648-669).
+bool Sim::launch_one(int slot, bool capturing) {
     if (opt_.morton && slot % kGraphBody == 0) {
         int M = 1;
         while (M < p_.num_cells) M <<= 1;
         PFH_OK_OR_FAIL(pfh2d_morton_sort(d_cell_, d_perm_, p_.num_cells, M, side_, stream_));
     }
-    const pfh2d_step_args a = args_for_slot(slot);
-    PFH_OK_OR_FAIL(pfh2d_launch_step(&a, grid_, stream_, nullptr, 0, 0.0f, 0));
+    const pfh2d_step_args a = args_for_slot(slot % kGraphBody);
+    int observation = PFH_OBSERVE_DISABLED;
+    if (velocity_.enabled()) {
+        if (velocity_.reference())
+            PFH_OK_OR_FAIL(pfh2d_velocity_moments(&a, velocity_.accumulators(), p_.dt, stream_));
+        else
+            observation = velocity_.sampling().spatial_sample(slot, steps_done_, capturing)
+                              ? PFH_OBSERVE_SPATIAL : PFH_OBSERVE_ADVECTION;
+    }
+    PFH_OK_OR_FAIL(pfh2d_launch_step(&a, grid_, stream_, nullptr, 0, 0.0f, observation));
+    if (!capturing) velocity_.sampling().advanced();
     return true;
 }
 
 bool Sim::build_graph() {
     hipStream_t s = (hipStream_t)stream_;
-    hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
-    bool ok = true;
-    for (int slot = 0; slot < kGraphBody && ok; ++slot) ok = launch_one(slot);
-    const hipError_t e = hipStreamEndCapture(s, &g);
-    if (!ok || e != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return false;
+    const VelocitySampling& sampling = velocity_.sampling();
+    const int steps = sampling.graph_steps();
+    for (int piece = 0; piece < sampling.graph_count(); ++piece) {
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+        bool ok = true;
+        for (int k = 0; k < steps && ok; ++k) ok = launch_one(piece * steps + k, true);
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (!ok || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        hipGraphExec_t x = nullptr;
+        if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return false;
+        }
+        graphs_.push_back(g);
+        graph_execs_.push_back(x);
     }
-    hipGraphExec_t x = nullptr;
-    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return false;
-    }
-    graph_ = g;
-    graph_exec_ = x;
-    std::printf("  HIP graph: %d steps cover phi parity and S rotation%s\n", kGraphBody,
+    std::printf("  HIP graphs: %d x %d steps cover phi parity, S rotation and the observation "
+                "cadence%s\n", sampling.graph_count(), steps,
                 opt_.morton ? "; Morton sort every six steps" : "");
     return true;
+}
+
+// One replay of the graph piece that starts at the current step, when the
+// sampling allows it and the piece ends at or before next_stop.
+bool Sim::replay_graph(bool* replayed, long long next_stop) {
+    VelocitySampling& sampling = velocity_.sampling();
+    const int steps = sampling.graph_steps();
+    *replayed = false;
+    if (graph_execs_.empty() || !sampling.can_replay(steps_done_) || steps_done_ % steps != 0 ||
+        steps_done_ + steps > next_stop)
+        return true;
+    const hipGraphExec_t x = (hipGraphExec_t)graph_execs_[sampling.graph_index(steps_done_)];
+    if (hipGraphLaunch(x, (hipStream_t)stream_) != hipSuccess) {
+        std::fprintf(stderr, "[hip] graph launch failed\n");
+        return false;
+    }
+    steps_done_ += steps;
+    sampling.advanced();
+    *replayed = true;
+    return true;
+}
+
+bool Sim::open_observations() {
+    return velocity_.open(args_for_slot((int)(steps_done_ % kGraphBody)), steps_done_, stream_);
+}
+
+bool Sim::close_observations(bool valid_state) {
+    bool ok = true;
+    if (valid_state && velocity_.enabled() && velocity_.last_frame() != steps_done_)
+        ok = velocity_.capture(args_for_slot((int)(steps_done_ % kGraphBody)), steps_done_,
+                               stream_);
+    if (!velocity_.close()) ok = false;
+    return ok;
 }
 
 void Sim::print_line() {
@@ -521,19 +574,19 @@ bool Sim::run() {
         !open_trajectory(opt_.out_path, p_, side_, steps_done_, trajectory_every_, &traj_frames_,
                          &traj_fp_))
         return false;
+    if (!open_observations()) return false;
+    const int period = velocity_.sampling().graph_period();
     hipStream_t s = (hipStream_t)stream_;
     while (steps_done_ < total) {
         const long long next_stop = std::min({total, (steps_done_ / pi + 1) * pi, next_traj,
                                               next_ckpt, next_save, next_poll});
-        if (graph_exec_ && steps_done_ % kGraphBody == 0 && steps_done_ + kGraphBody <= next_stop) {
-            if (hipGraphLaunch((hipGraphExec_t)graph_exec_, s) != hipSuccess) {
-                std::fprintf(stderr, "[hip] graph launch failed\n");
-                failed = true;
-                break;
-            }
-            steps_done_ += kGraphBody;
-        } else {
-            if (!launch_one((int)(steps_done_ % kGraphBody))) {
+        bool replayed = false;
+        if (!replay_graph(&replayed, next_stop)) {
+            failed = true;
+            break;
+        }
+        if (!replayed) {
+            if (!launch_one((int)(steps_done_ % period))) {
                 failed = true;
                 break;
             }
@@ -560,6 +613,13 @@ bool Sim::run() {
                     break;
                 }
                 next_traj = next_boundary(steps_done_, traj_every);
+                // a velocity frame at every trajectory frame
+                if (velocity_.enabled() &&
+                    !velocity_.capture(args_for_slot((int)(steps_done_ % kGraphBody)),
+                                       steps_done_, stream_)) {
+                    failed = true;
+                    break;
+                }
             }
             if (do_ckpt || do_save) {
                 if (!save_checkpoint(checkpoint_paths(do_ckpt, do_save))) {
@@ -601,6 +661,7 @@ bool Sim::run() {
         }
         traj_fp_ = nullptr;
     }
+    if (!close_observations(!failed)) failed = true;
     if (ckpt_on && (opt_.final_checkpoint || failed)) {
         const std::vector<std::string> paths =
             failed ? std::vector<std::string>{opt_.ckpt_dir + "/checkpoint_failed.bin"}
@@ -620,13 +681,15 @@ bool Sim::run() {
 
 bool Sim::bench(int steps, double* ms_per_step) {
     hipStream_t s = (hipStream_t)stream_;
+    const int period = velocity_.sampling().graph_period();
+    const int gsteps = velocity_.sampling().graph_steps();
     const int warm = std::min(steps, 200);
     for (int i = 0; i < warm; ++i) {
-        if (!launch_one((int)(steps_done_ % kGraphBody))) return false;
+        if (!launch_one((int)(steps_done_ % period))) return false;
         ++steps_done_;
     }
-    while (steps_done_ % kGraphBody != 0) {
-        if (!launch_one((int)(steps_done_ % kGraphBody))) return false;
+    while (steps_done_ % gsteps != 0) {
+        if (!launch_one((int)(steps_done_ % period))) return false;
         ++steps_done_;
     }
     HIP_OK(hipStreamSynchronize(s));
@@ -636,11 +699,13 @@ bool Sim::bench(int steps, double* ms_per_step) {
     const long long from = steps_done_;
     HIP_OK(hipEventRecord(e0, s));
     while (steps_done_ - from < steps) {
-        if (graph_exec_) {
-            HIP_OK(hipGraphLaunch((hipGraphExec_t)graph_exec_, s));
-            steps_done_ += kGraphBody;
+        if (!graph_execs_.empty()) {
+            const int g = velocity_.sampling().graph_index(steps_done_);
+            HIP_OK(hipGraphLaunch((hipGraphExec_t)graph_execs_[g], s));
+            steps_done_ += gsteps;
+            velocity_.sampling().advanced();
         } else {
-            if (!launch_one((int)(steps_done_ % kGraphBody))) return false;
+            if (!launch_one((int)(steps_done_ % period))) return false;
             ++steps_done_;
         }
     }
@@ -654,7 +719,7 @@ bool Sim::bench(int steps, double* ms_per_step) {
     (void)hipEventDestroy(e1);
     std::printf("bench: %lld steps in %.3f ms -> %.6f ms/step (%.2f us/step, %d cells, L=%d, "
                 "%s)\n", done, (double)ms, *ms_per_step, *ms_per_step * 1000.0, p_.num_cells,
-                side_, graph_exec_ ? "graph" : "per-step launch");
+                side_, graph_execs_.empty() ? "per-step launch" : "graph");
     report_flags();
     return true;
 }

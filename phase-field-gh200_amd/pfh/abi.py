@@ -44,6 +44,16 @@ CELL_DTYPE = np.dtype({
     "itemsize": 192,
 })
 
+# MomentAccum (common/velocity_moments.h:30-40): 240 bytes.
+MOMENT_DTYPE = np.dtype([("q", "<f8", (12,)), ("count", "<u8"), ("held", "<f8", (6,)),
+                         ("spatial_count", "<u8"), ("last_sample", "<u8"),
+                         ("last_interval", "<u8"), ("last_totals", "<f8", (4,)),
+                         ("correction", "<f8", (4,))])
+assert MOMENT_DTYPE.itemsize == 240
+
+# Velocity observation modes of pfh2d_launch_step
+OBSERVE_DISABLED, OBSERVE_ADVECTION, OBSERVE_SPATIAL = 0, 1, 2
+
 # TrajPackedCell (include/kernels.cuh:60-70): 48 bytes.
 TRAJ_DTYPE = np.dtype([("global_id", "<i4"), ("cls", "<i4"), ("cx", "<f4"),
                        ("cy", "<f4"), ("vx", "<f4"), ("vy", "<f4"),
@@ -219,6 +229,7 @@ SIGNATURES = {
     "pfh2d_configure_morton": (_I, [_I]),
     "pfh2d_step_grid": (_I, [_I]),
     "pfh2d_launch_step": (_I, [C.POINTER(StepArgs), _I, _P, _P, _SZ, _F, _I]),
+    "pfh2d_velocity_moments": (_I, [C.POINTER(StepArgs), _P, C.c_double, _P]),
     "pfh2d_launch_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
     "pfh2d_launch_check_step_if": (_I, [C.POINTER(StepArgs), _P, _I, _P]),
     "pfh2d_launch_batch_step": (_I, [C.POINTER(StepArgs), _P, _P, _I, _I, _I, _I, _P]),

@@ -56,6 +56,7 @@ class Sim2D:
             abi.check(self.grid, "pfh2d_step_grid")
         abi.check(self.lib.pfh2d_configure_step(), "pfh2d_configure_step")
         self.steps_done = 0
+        self.moments = None
         self._args = [self._bake(s) for s in range(GRAPH_BODY)]
         self._graph = None
         self._init_fresh(wl.cx, wl.cy)
@@ -158,14 +159,35 @@ class Sim2D:
         self.flags.zero_()
         torch.cuda.synchronize(self.dev)
 
-    # ---- stepping ----
-    def launch_one(self, slot: int) -> None:
-        abi.check(self.lib.pfh2d_launch_step(C.byref(self._args[slot]), self.grid, self._sp(),
-                                             None, 0, 0.0, 0), "pfh2d_launch_step")
+    # ---- velocity observation (src/sim.cu:648-669) ----
+    def attach_moments(self) -> torch.Tensor:
+        """Allocate the per-cell moment accumulators (zeroed) that spatial
+        observation writes, and point every baked argument set at them."""
+        self.moments = torch.zeros(self.N * abi.MOMENT_DTYPE.itemsize, dtype=torch.uint8,
+                                   device=self.dev)
+        for a in self._args:
+            a.moments = _ptr(self.moments)
+        return self.moments
 
-    def step(self, n: int = 1) -> None:
+    def moments_host(self) -> np.ndarray:
+        return self.moments.cpu().numpy().view(abi.MOMENT_DTYPE)
+
+    def observe_dense(self, out: torch.Tensor, physical_dt: float | None = None) -> None:
+        """The independent dense observer on the next step's input
+        (pfh2d_velocity_moments) into out (N accumulators)."""
+        a = self._args[self.steps_done % GRAPH_BODY]
+        abi.check(self.lib.pfh2d_velocity_moments(
+            C.byref(a), _ptr(out), self.p.dt if physical_dt is None else physical_dt,
+            self._sp()), "pfh2d_velocity_moments")
+
+    # ---- stepping ----
+    def launch_one(self, slot: int, observation: int = abi.OBSERVE_DISABLED) -> None:
+        abi.check(self.lib.pfh2d_launch_step(C.byref(self._args[slot]), self.grid, self._sp(),
+                                             None, 0, 0.0, observation), "pfh2d_launch_step")
+
+    def step(self, n: int = 1, observation: int = abi.OBSERVE_DISABLED) -> None:
         for _ in range(n):
-            self.launch_one(self.steps_done % GRAPH_BODY)
+            self.launch_one(self.steps_done % GRAPH_BODY, observation)
             self.steps_done += 1
 
     def capture_graph(self) -> None:

@@ -48,8 +48,12 @@ def test_invalid_arguments_rejected_without_gpu():
     assert lib.pfh2d_launch_step(None, 1, None, None, 0, 0.0, 0) == -1
     a = abi.StepArgs()
     assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 0) == -1
-    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 1) == -3   # observer
+    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 3) == -1   # no such mode
     assert b"observation" in lib.pfh_last_error()
+    # a spatial observation without accumulators
+    assert lib.pfh2d_launch_step(C.byref(a), 1, None, None, 0, 0.0, 2) == -1
+    assert b"moments" in lib.pfh_last_error()
+    assert lib.pfh2d_velocity_moments(C.byref(a), None, 0.01, None) == -1
     assert lib.pfh2d_init_tiles(None, None, None, None, 4, 64, None, None, 1.0, 1.0, None) == -1
     assert lib.pfh2d_init_moments(None, None, None, -1, None) == -1
     assert lib.pfh2d_scatter_all(None, None, None, None, 2, 100, 50, None, None) == -1

@@ -159,3 +159,65 @@ def test_graph_replay_and_morton_order_leave_state_unchanged(tmp_path):
         out[name] = (d / "ck" / "checkpoint.bin").read_bytes()
     assert out["graph"] == out["eager"]
     assert out["morton"] == out["eager"]
+
+
+def _read_pfvmom4(path):
+    """PFVMOM4 (common/velocity_format.h): 56-byte header, then frames of an
+    int64 step and cells x columns float64."""
+    raw = Path(path).read_bytes()
+    magic, cells, cols, dt, start, side, stride, quad, dense = struct.unpack(
+        "<8sIIdqdIIQ", raw[:56])
+    assert magic == b"PFVMOM4\0" and cols == 26 and quad == 2
+    frame = 8 + 8 * cells * cols
+    assert (len(raw) - 56) % frame == 0
+    frames = []
+    for k in range((len(raw) - 56) // frame):
+        off = 56 + k * frame
+        step = struct.unpack("<q", raw[off:off + 8])[0]
+        rows = np.frombuffer(raw[off + 8:off + frame], np.float64).reshape(cells, cols)
+        frames.append((step, rows))
+    return dict(cells=cells, dt=dt, start=start, side=side, stride=stride, dense=dense,
+                frames=frames)
+
+
+def test_velocity_moments_fused_equal_dense_reference(tmp_path):
+    """The PFVMOM4 output of a run observed by the fused step variants
+    (spatial moments every step) and of the same run observed by the
+    independent dense observer (--velocity-moments-reference) integrate the
+    same velocity components: frames at the same steps, same cell columns,
+    cumulative integrals within 1e-9 relative. A run with a spatial stride
+    of 5 after a 10-step dense start records every step, and its physics is
+    unchanged by the observation (its final checkpoint equals an unobserved
+    run's)."""
+    base = ["--N", 16, "--radius", 20, "--rho", 0.24, "--seed", 3, "--polarity-seed", 4,
+            "--tau", 0.5, "--t-end", 0.3, "--print-interval", 0, "--trajectory-interval", 10]
+    pf2d(base + ["--out", "a.txt", "--velocity-moments", "a.vm", "--velocity-spatial-stride", 1,
+                 "--velocity-dense-start", 100], tmp_path)
+    pf2d(base + ["--out", "b.txt", "--velocity-moments", "b.vm",
+                 "--velocity-moments-reference"], tmp_path)
+    pf2d(base + ["--out", "c.txt", "--velocity-moments", "c.vm", "--velocity-spatial-stride", 5,
+                 "--velocity-dense-start", 10, "--checkpoint-dir", "ckc"], tmp_path)
+    pf2d(base + ["--out", "d.txt", "--checkpoint-dir", "ckd"], tmp_path)
+    a, b, c = (_read_pfvmom4(tmp_path / f) for f in ("a.vm", "b.vm", "c.vm"))
+    assert a["cells"] == 16 and a["stride"] == 1 and b["stride"] == 1 and c["stride"] == 5
+    steps = [s for s, _ in a["frames"]]
+    assert steps == [0, 10, 20, 30] == [s for s, _ in b["frames"]] == [s for s, _ in c["frames"]]
+    for (s, ra), (_, rb), (_, rc) in zip(a["frames"], b["frames"], c["frames"]):
+        assert np.isfinite(ra).all() and np.isfinite(rb).all()
+        np.testing.assert_array_equal(ra[:, :9], rb[:, :9])        # id, centroid, ..., count
+        assert (ra[:, 8] == s).all()
+        q_a, q_b = ra[:, 9:21], rb[:, 9:21]
+        # q columns 8, 9: the area source's moment, exactly zero for the
+        # fused observer and rounding noise for the dense one
+        cols = [0, 1, 2, 3, 4, 5, 6, 7, 10, 11]
+        scale = np.abs(q_b[:, cols]).max(axis=0) + 1e-300
+        assert (np.abs(q_a[:, cols] - q_b[:, cols]) / scale).max() < 1e-9, s
+        # (cancelling sums: their noise is relative to the sum of magnitudes)
+        assert np.abs(q_a[:, 8:10] - q_b[:, 8:10]).max() <= 1e-8 * np.abs(q_b[:, 4:8]).max()
+        np.testing.assert_array_equal(rc[:, :9], ra[:, :9])
+        assert (rc[:, 21] <= ra[:, 21]).all()                       # fewer spatial samples
+    assert (a["frames"][-1][1][:, 21] == 30).all()
+    # observation changes nothing in the physics
+    assert (tmp_path / "ckc" / "checkpoint.bin").read_bytes() == \
+        (tmp_path / "ckd" / "checkpoint.bin").read_bytes()
+    assert (tmp_path / "c.txt").read_bytes() == (tmp_path / "d.txt").read_bytes()

@@ -38,8 +38,16 @@ def test_help_and_bad_options(tmp_path):
     assert pf2d(["--help"], tmp_path).returncode == 0
     r = pf2d(["--bogus"], tmp_path)
     assert r.returncode == 2 and "unknown option" in r.stdout
-    r = pf2d(["--velocity-moments", "v.bin"], tmp_path)
+    r = pf2d(["--boundary-out", "b.bin"], tmp_path)
     assert r.returncode == 2 and "not supported" in r.stdout
+    # the reference's rule: velocity recording needs an output path and --out
+    # (src/main.cu:512-517)
+    r = pf2d(["--velocity-moments", "v.bin"], tmp_path)
+    assert r.returncode == 2 and "requires --velocity-moments and --out" in r.stdout
+    r = pf2d(["--velocity-spatial-stride", "5", "--out", "t.txt"], tmp_path)
+    assert r.returncode == 2 and "requires --velocity-moments and --out" in r.stdout
+    r = pf2d(["--velocity-spatial-stride", "1001"], tmp_path)
+    assert r.returncode == 2
     r = pf2d(["--N", "0"], tmp_path)
     assert r.returncode == 2
     r = pf2d(["--N", "4", "--rho", "0.5"], tmp_path)   # L = 18: smaller than a window
