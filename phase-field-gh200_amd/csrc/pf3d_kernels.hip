@@ -682,8 +682,11 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     const int wyv = row_ok ? wrap_off(oy, syv, L) : 0;
     const PFH_GLOBAL uint32_t* Sin = gptr(a.S_in);
     // S words of iteration i's destination row (source centre plane pbeg + i - 1),
-    // fetched one iteration ahead; indices exactly those the update reads
-    auto load_su = [&](int i, uint32_t (&sv)[4]) {
+    // fetched one iteration ahead; indices exactly those the update reads, and
+    // only where the voxel's own phi (cv, the source centre value) is nonzero:
+    // at phi = 0 the interaction term ic phi S_other is a zero of the same
+    // sign for any finite S_other, and S_other cannot be negative
+    auto load_su = [&](int i, const float (&cv)[4], uint32_t (&sv)[4]) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) sv[j] = 0u;
         const int zc = pbeg + i - 1;
@@ -692,12 +695,16 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int xs = gx0 + j + sx;
-                if ((unsigned)xs < (unsigned)B) sv[j] = Sin[srow + (size_t)wrap_off(ox, xs, L)];
+                if ((unsigned)xs < (unsigned)B && cv[j] != 0.0f)
+                    sv[j] = Sin[srow + (size_t)wrap_off(ox, xs, L)];
             }
         }
     };
     uint32_t Su[4], Sun[4];
-    load_su(0, Su);
+    {
+        const float none[4] = {0.f, 0.f, 0.f, 0.f};
+        load_su(0, none, Su);
+    }
     // S_OVERFLOW test of the previous plane's scatters, taken one plane later
     // so that no atomic's return is waited for in the plane that issued it
     uint32_t pold[4] = {0u, 0u, 0u, 0u}, pq[4] = {0u, 0u, 0u, 0u};
@@ -714,7 +721,6 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
         const int p = pbeg + i;
         const int cur = i & 1;
         const f32x4 Rnn = (i + 2 < pcount) ? load_plane(p + 2) : f32x4{0.f, 0.f, 0.f, 0.f};
-        load_su(i + 1, Sun);
         float A[4] = {0.f, 0.f, 0.f, 0.f}, Q1[4] = {0.f, 0.f, 0.f, 0.f},
               Q0[4] = {0.f, 0.f, 0.f, 0.f}, GX[4] = {0.f, 0.f, 0.f, 0.f},
               GY[4] = {0.f, 0.f, 0.f, 0.f};
@@ -734,6 +740,8 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                 A[j] = s.A; Q1[j] = s.q1; Q0[j] = s.q0; GX[j] = s.gx; GY[j] = s.gy;
             }
         }
+        // A holds this plane's centre values: the next destination row's phi
+        load_su(i + 1, A, Sun);
         check_overflow();
         if (i >= 2 && act) {
             // destination plane d = p - 1 - sz from source centre plane p - 1
