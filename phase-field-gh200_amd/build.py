@@ -1,9 +1,10 @@
-"""Build recipe of the gfx950 library (lib/libpfh.so) and the pf2d driver.
+"""Build recipe of the gfx950 library (lib/libpfh.so) and the two drivers.
 
 Compiles every HIP source under csrc/ for gfx950 into one shared library
-exporting the C ABI of include/pfh.h, then the C++ host driver under host/
-(the pf2d command-line simulator) into lib/pf2d, linked against that
-library. Runs on a machine without a GPU (hipcc cross-compiles).
+exporting the C ABI of include/pfh.h, then the C++ host drivers under host/
+(the pf2d and pf3d command-line simulators) into lib/pf2d and lib/pf3d,
+linked against that library. Runs on a machine without a GPU (hipcc
+cross-compiles).
 """
 from __future__ import annotations
 
@@ -55,8 +56,8 @@ def _stale(out: Path, deps: list[Path]) -> bool:
 
 
 def build(force: bool = True) -> Path:
-    """Compile csrc/ into lib/libpfh.so and host/ into lib/pf2d. Returns the
-    .so path."""
+    """Compile csrc/ into lib/libpfh.so and host/ into lib/pf2d and lib/pf3d.
+    Returns the .so path."""
     LIB.mkdir(exist_ok=True)
     hip_srcs = sorted(CSRC.glob("*.hip"))
     cpp_srcs = sorted(CSRC.glob("*.cpp"))
@@ -74,15 +75,18 @@ def build(force: bool = True) -> Path:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
           *map(str, objs)])
     os.replace(tmp, so)
-    # The host driver: host-only C++ that reaches the GPU through the C ABI.
-    host_srcs = sorted((HERE / "host").glob("*.cpp"))
+    # The host drivers: host-only C++ that reaches the GPU through the C ABI
+    # (host/pf2d_*.cpp -> lib/pf2d, host/pf3d_*.cpp -> lib/pf3d).
     host_hdrs = sorted((HERE / "host").glob("*.hpp")) + [REPO / "include" / "pfh.h"]
-    exe = LIB / "pf2d"
-    if host_srcs and (force or _stale(exe, host_srcs + host_hdrs + [so])):
-        tmp_exe = LIB / "pf2d.tmp"
-        _run([HIPCC, "-O2", "-std=c++17", "-I", str(REPO / "include"), *map(str, host_srcs),
-              "-o", str(tmp_exe), "-L", str(LIB), "-lpfh", "-Wl,-rpath,$ORIGIN"])
-        os.replace(tmp_exe, exe)
+    for name in ("pf2d", "pf3d"):
+        host_srcs = sorted((HERE / "host").glob(f"{name}_*.cpp"))
+        exe = LIB / name
+        if host_srcs and (force or _stale(exe, host_srcs + host_hdrs + [so])):
+            tmp_exe = LIB / f"{name}.tmp"
+            _run([HIPCC, "-O2", "-std=c++17", "-I", str(REPO / "include"),
+                  *map(str, host_srcs), "-o", str(tmp_exe), "-L", str(LIB), "-lpfh",
+                  "-Wl,-rpath,$ORIGIN"])
+            os.replace(tmp_exe, exe)
     return so
 
 
