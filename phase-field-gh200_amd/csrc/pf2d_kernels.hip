@@ -65,6 +65,7 @@ struct Smem {
     uint32_t csraw[2][48];       // CellState (192 B) of the next cell
     const pfh2d_replica* reps;   // batched step only: replica table
     const uint16_t* cell_rep;    // batched step only: replica of each cell
+    int fm;                      // full-moment step (one in full_moment_every)
 };
 
 // The staged window, an LDS object of its own: the global-to-LDS window
@@ -123,10 +124,11 @@ struct ScalarArgs {
     unsigned long long polarity_seed;
     double p_tumble, vol_scale, A0;
     float bulk_scale;
-    int L, parity_out, full_moment_every;
+    int L, parity_out;
+    int fm;              // this launch's step is a full-moment step
 };
 
-__device__ __forceinline__ ScalarArgs scalar_args(const pfh2d_step_args& A) {
+__device__ __forceinline__ ScalarArgs scalar_args(const pfh2d_step_args& A, int fm) {
     ScalarArgs s;
     s.cell = A.cell;
     s.flags = A.flags;
@@ -137,7 +139,7 @@ __device__ __forceinline__ ScalarArgs scalar_args(const pfh2d_step_args& A) {
     s.bulk_scale = A.bulk_scale;
     s.L = A.L;
     s.parity_out = A.parity_out;
-    s.full_moment_every = A.full_moment_every;
+    s.fm = fm;
     return s;
 }
 
@@ -166,9 +168,12 @@ __device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_sta
     const float dwC = A.bulk_scale * gam;
     const float volC = (float)(A.vol_scale * (A.A0 - cs.V));
     const float mrel = cs.M_pf * (float)(1.0 / 0.5);
+    // the centroid (also the observers' reference point)
+    const double cxr = cs.Cx / cs.V;
+    const double cyr = cs.Cy / cs.V;
     ctl->og2iv = 2.0 / cs.V;
-    ctl->ocx = cs.Cx / cs.V;
-    ctl->ocy = cs.Cy / cs.V;
+    ctl->ocx = cxr;
+    ctl->ocy = cyr;
 
     const int ex = cs.bb_hi_x - cs.bb_lo_x + 1;
     const int ey = cs.bb_hi_y - cs.bb_lo_y + 1;
@@ -202,8 +207,6 @@ __device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_sta
     const Shape dc = shape_of(dcls);
     int sx = 0, sy = 0;
     if (cs.V > 0.0) {
-        const double cxr = cs.Cx / cs.V;
-        const double cyr = cs.Cy / cs.V;
         sx = __double2int_rn(cxr - 0.5 * (double)(dc.wx - 1));
         sy = __double2int_rn(cyr - 0.5 * (double)(dc.wy - 1));
         if (dcls == CLS) {
@@ -224,8 +227,7 @@ __device__ __noinline__ void cell_scalars(const ScalarArgs A, const pfh_cell_sta
     ctl->gy0n = wrapi(cs.gy0 + sy, A.L);
     ctl->dcls = dcls;
     ctl->zero_tile = (dcls != (int)cs.cls_written[A.parity_out]);
-    ctl->fm = (A.full_moment_every > 0 &&
-               ((step + 1ull) % (unsigned long long)A.full_moment_every) == 0ull);
+    ctl->fm = A.fm;
     ctl->pctr = (int)pctr;
     ctl->tumbled = tumbled;
     ctl->dwx = dc.wx;
@@ -594,7 +596,7 @@ __device__ __forceinline__ void fetch4b(const pfh2d_step_args& A, Smem& sm, int 
                                         unsigned long long step) {
     const int cls = sm.slot[which].cls;
     if (cls >= 0 && cls < kNumClasses) {
-        ScalarArgs s = scalar_args(A);
+        ScalarArgs s = scalar_args(A, sm.fm);
         if constexpr (kBatch) {
             // batched step: the cell's replica sets its domain and stream
             Slot& sl = sm.slot[which];
@@ -735,7 +737,7 @@ __device__ __forceinline__ Dom cell_dom(const pfh2d_step_args& A, const Slot& c)
 template <int CLS, bool kCheck = false, bool kBatch = false, int kObs = 0>
 __device__ void cell_staged(const pfh2d_step_args& A, int n,
                             unsigned long long step, Smem& sm, Ctl& c, int nxt,
-                            int& pre, int& pslot) {
+                            int& pre, int& pslot, int* ring = nullptr) {
     using G = Staged<CLS>;
     constexpr int WX = G::WX, WY = G::WY, TX0 = G::TX0, TY0 = G::TY0;
     constexpr int PX = G::PX;
@@ -848,18 +850,31 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     }
     // zero ring: the pads of every row (column -1 and column WX) and rows -1, WY
     if constexpr (CLS == 0) {
-        constexpr int n1 = (Split0::P1_ROWS + 1) * 4, n2 = (Split0::P2_ROWS + 1) * 4;
-        for (int i = tid; i < n1 + n2 + 2 * WX; i += kNT) {
-            if (i < n1) {
-                base1[(i >> 2) * PX + (i & 3)] = 0.0f;
-            } else if (i < n1 + n2) {
-                const int j = i - n1;
-                base2[(j >> 2) * PX + (j & 3)] = 0.0f;
-            } else {
-                const int j = i - n1 - n2;
-                if (j < WX) base1[4 + j] = 0.0f;
-                else base2[(WY - Split0::P2_R0) * PX + 4 + (j - WX)] = 0.0f;
+        // The class-0 layout's ring is written by nothing but this set-up
+        // (copies, staging and the perimeter pass write data columns only):
+        // both piece-1 slots and piece 2 are set up once, and again after a
+        // cell of another class has used the LDS.
+        if (*ring == 0) {
+            constexpr int n1 = (Split0::P1_ROWS + 1) * 4, n2 = (Split0::P2_ROWS + 1) * 4;
+            float* const b2 = phi_s + 2 * Split0::P1_FLOATS;
+            for (int i = tid; i < 2 * n1 + n2 + 3 * WX; i += kNT) {
+                if (i < 2 * n1) {
+                    const int sl = i >= n1 ? 1 : 0, j = i - sl * n1;
+                    phi_s[sl * Split0::P1_FLOATS + (j >> 2) * PX + (j & 3)] = 0.0f;
+                } else if (i < 2 * n1 + n2) {
+                    const int j = i - 2 * n1;
+                    b2[(j >> 2) * PX + (j & 3)] = 0.0f;
+                } else {
+                    const int j = i - 2 * n1 - n2;
+                    if (j < 2 * WX) {
+                        const int sl = j >= WX ? 1 : 0;
+                        phi_s[sl * Split0::P1_FLOATS + 4 + (j - sl * WX)] = 0.0f;
+                    } else {
+                        b2[(WY - Split0::P2_R0) * PX + 4 + (j - 2 * WX)] = 0.0f;
+                    }
+                }
             }
+            *ring = 1;
         }
     } else {
         for (int i = tid; i < (WY + 3) * 4 + 2 * WX; i += kNT) {
@@ -1322,6 +1337,8 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     const unsigned long long step = *A.step_rd;
     if (tid == 0) {
         sm.args = A;
+        sm.fm = (A.full_moment_every > 0 &&
+                 ((step + 1ull) % (unsigned long long)A.full_moment_every) == 0ull) ? 1 : 0;
         if constexpr (kBatch) {
             sm.reps = reps;
             sm.cell_rep = cell_rep;
@@ -1336,6 +1353,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
     int cur = 0;
     int pre = 0;                       // the current class-0 window is already in LDS
     int pslot = 0;                     // slot of the class-0 window's piece 1
+    int ring = 0;                      // the class-0 layout's zero ring is set up
     for (;;) {
         const int cls = __builtin_amdgcn_readfirstlane(sm.slot[cur].cls);
         const int n = __builtin_amdgcn_readfirstlane(sm.slot[cur].n);
@@ -1343,7 +1361,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         switch (cls) {
             case 0:
                 cell_staged<0, kCheck, kBatch, kObs>(A, n, step, sm, sm.slot[cur].ctl, cur ^ 1,
-                                                     pre, pslot);
+                                                     pre, pslot, &ring);
                 break;
             case 1: cell_cold<1, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
             case 2: cell_cold<2, kBatch, kObs>(&sm.args, n, step, &sm, cur); break;
@@ -1357,6 +1375,7 @@ __global__ __launch_bounds__(kNT, 1) void k_step(const pfh2d_step_args A, const 
         }
         if (cls != 0) {
             pre = 0;
+            ring = 0;
             // Retire the out-of-line paths' register reloads here, so that the
             // loop head does not wait on them (with everything else in flight).
             __builtin_amdgcn_s_waitcnt(0);
