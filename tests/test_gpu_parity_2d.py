@@ -91,6 +91,28 @@ def test_c1_100_steps_bit_exact_and_tolerances(sim_mod, oracle):
     assert not sim.fatal()
 
 
+def test_c1_1000_steps_bit_exact(sim_mod, oracle):
+    """The 384^2 / 16-cell workload over its full 1000 steps: bit-exact
+    against the oracle in the kernel's order every 250 steps, and within
+    1e-5 max-abs of the FP64 oracle at the end."""
+    wl = params.workload_c1()
+    say("C1 1000 steps vs oracle (kernel order every 250 steps, FP64 at the end)")
+    sim = sim_mod.Sim2D(wl)
+    sim.capture_graph()
+    orc = oracle.OracleSim2D(wl, threads=8)
+    for k in range(4):
+        sim.run(250)
+        orc.step(250)
+        sim.sync()
+        assert_state_equal(sim, orc, f"C1 step {sim.steps_done}")
+    f64 = oracle.OracleSim2D(wl, f64=True, threads=8)
+    f64.step(1000)
+    err = float(np.abs(sim.phi_now().astype(np.float64) - f64.phi_now()).max())
+    say(f"C1 1000 steps: max|phi - FP64| = {err:.3e}")
+    assert err <= 1e-5
+    assert not sim.fatal()
+
+
 def _forced_state(oracle, L=701, seed=3):
     """All six shape classes, forced tumbles, heterogeneous stiffness and
     mobility, and scripted class changes and fatal conditions."""
@@ -253,17 +275,18 @@ def test_c2_parity_and_S_exactness(sim_mod, oracle):
 
 
 def test_c2_long_run_S_exact(sim_mod, oracle):
-    """1002 graph-replayed steps of the 2048^2 / 1024-cell monolayer: the
-    aggregate S still equals the oracle's scatter of the stored fields bit for
-    bit (no drift from the incremental scatter and clear-ahead rotation), no
-    fatal flag is raised and the field stays finite and bounded."""
+    """The 2048^2 / 1024-cell monolayer over its full 10,000 steps (10,002:
+    whole six-step graphs): the aggregate S still equals the oracle's scatter
+    of the stored fields bit for bit (no drift from the incremental scatter
+    and clear-ahead rotation), no fatal flag is raised and the field stays
+    finite and bounded."""
     wl = params.workload_c2()
     sim = sim_mod.Sim2D(wl)
     sim.capture_graph()
-    say("C2: 1002 steps from six-step graphs")
-    sim.run(1002)
+    say("C2: 10002 steps from six-step graphs")
+    sim.run(10002)
     sim.sync()
-    assert sim.steps_done == 1002
+    assert sim.steps_done == 10002
     assert not sim.fatal(), sim.flag_counts()
     phi = np.ascontiguousarray(sim.phi_now())
     assert np.isfinite(phi).all()

@@ -130,17 +130,19 @@ def test_c3_steps_bit_exact(sim3, o3):
 
 
 def test_c3_long_run_S_exact(sim3, o3):
-    """300 graph-replayed steps of the 256^3 / 128-cell workload (brick 56):
-    the aggregate S equals the oracle's scatter of the stored fields at the
+    """The 256^3 / 128-cell workload (brick 56) over its full 2000 steps from
+    graphs: no flag at all (the brick holds the relaxed support), the
+    aggregate S equals the oracle's scatter of the stored fields at the
     stored brick origins bit for bit (no drift over many recentring shifts),
-    no fatal flag, and the field stays finite and bounded."""
+    and the field stays finite and bounded."""
     wl = params3d.workload_c3()
     sim = sim3.Sim3D(wl)
     sim.capture_graph()
-    say(f"{wl.name}: 300 steps from graphs")
-    sim.run(300)
+    say(f"{wl.name}: 2000 steps from graphs")
+    sim.run(2000)
     sim.sync()
-    assert not sim.fatal(), sim.flag_counts()
+    assert sim.steps_done == 2000
+    assert not sim.flag_counts().any(), sim.flag_counts()
     phi = np.ascontiguousarray(sim.phi_now())
     assert np.isfinite(phi).all()
     assert phi.max() <= 1.5 and phi.min() >= -0.5 and phi.max() > 0.9
