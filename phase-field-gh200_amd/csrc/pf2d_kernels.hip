@@ -661,15 +661,39 @@ __device__ __forceinline__ bool wave_any(bool v) {
 }
 
 // First part of phase_field_rhs (include/kernels.cuh:127-136): every passive
-// term but the repulsion, in source order.
+// term but the repulsion, the reference's terms in its order with each
+// product-and-add fused (one rounding where the reference's source has a
+// multiply and an add): the 9-point Laplacian (edge sum x 4 + diagonal sum,
+// then - 20 x centre, over 6), then gam x lap - dwC x phi(1 - phi)(1 - 2 phi),
+// then + volC x phi.
+__device__ __forceinline__ float passive_head_of(float nC, float sC, float cE, float cW,
+                                                 float nE, float nW, float sE, float sW,
+                                                 float cC, float gam, float dwC, float volC) {
+    const float edge = nC + sC + cE + cW;
+    const float diag = nE + nW + sE + sW;
+    const float lap = fmaf(-20.0f, cC, fmaf(4.0f, edge, diag)) * (float)(1.0 / 6.0);
+    const float dw = (cC * (1.0f - cC)) * fmaf(-2.0f, cC, 1.0f);
+    return fmaf(volC, cC, fmaf(gam, lap, -(dwC * dw)));
+}
+
 __device__ __forceinline__ float passive_head(const Roll& q, float gam, float dwC,
                                               float volC) {
-    const float lap = (4.0f * (q.nC + q.sC + q.cE + q.cW)
-                       + 1.0f * (q.nE + q.nW + q.sE + q.sW)
-                       + -20.0f * q.cC) * (float)(1.0 / 6.0);
-    return gam * lap
-         - dwC * (q.cC * (1.0f - q.cC) * (1.0f - 2.0f * q.cC))
-         + volC * q.cC;
+    return passive_head_of(q.nC, q.sC, q.cE, q.cW, q.nE, q.nW, q.sE, q.sW, q.cC, gam, dwC,
+                           volC);
+}
+
+// The repulsion term: passive = head - repC x phi x S_other, fused.
+__device__ __forceinline__ float with_repulsion(float head, float repC, float cC, float So) {
+    return fmaf(-(repC * cC), So, head);
+}
+
+// The Euler update phi + dt (mrel x passive - (vx gx + vy gy)), fused. The
+// reference takes passive - adv when mrel == 1 (include/kernels.cuh:118-138):
+// 1 x passive is passive exactly, so one expression gives both.
+__device__ __forceinline__ float euler_update(float cC, float gxv, float gyv, float pas,
+                                              float vx, float vy, float mrel, float dt) {
+    const float adv = fmaf(vy, gyv, vx * gxv);
+    return fmaf(dt, fmaf(mrel, pas, -adv), cC);
 }
 
 __device__ __forceinline__ float s_other_of(uint32_t qS, float c, uint32_t& negc) {
@@ -919,7 +943,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                     observe_pixel(ob, x, y0 + k, q.cC, q.cE, q.cW, q.nC, q.sC, q.nE, q.nW,
                                   q.sE, q.sW, So, gam, dwC, repC, c.og2iv, c.ocx, c.ocy);
                 if (wave_any(So != 0.0f)) {
-                    pas[k] = head - repC * q.cC * So;
+                    pas[k] = with_repulsion(head, repC, q.cC, So);
                     const float gxv = 0.5f * (q.cE - q.cW);
                     const float gyv = 0.5f * (q.nC - q.sC);
                     aIx += (double)(q.cC * gxv * So);
@@ -960,12 +984,7 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
     int blox = dwx, bhix = -1, bloy = dwy, bhiy = -1, pmaxb = 0;
     bool nonfin = false, clampf = false, ovf = false;
     auto update = [&](float cC, float gxv, float gyv, float pas_k) -> float {
-        const float adv = vx * gxv + vy * gyv;
-        // the reference takes passive - adv when mrel == 1 and mrel * passive
-        // - adv otherwise (include/kernels.cuh:118-138); 1.0f * x is x exactly,
-        // so one expression gives both without a per-pixel select
-        const float rhs = mrel * pas_k - adv;
-        return cC + dt * rhs;
+        return euler_update(cC, gxv, gyv, pas_k, vx, vy, mrel, dt);
     };
     const int a = x - sx;
     if (active) {
@@ -1089,7 +1108,8 @@ __device__ void cell_staged(const pfh2d_step_args& A, int n,
                 if (row_ok(k)) {
                     load_row(cbw + (k + 2) * PX, q.nW, q.nC, q.nE);
                     const float So = s_other_of(qS2[k], q.cC, negc2);
-                    const float pk = passive_head(q, gam, dwC, volC) - repC * q.cC * So;
+                    const float pk = with_repulsion(passive_head(q, gam, dwC, volC), repC,
+                                                    q.cC, So);
                     pns[k] = update(q.cC, 0.5f * (q.cE - q.cW), 0.5f * (q.nC - q.sC), pk);
                     q.sW = q.cW; q.sC = q.cC; q.sE = q.cE;
                     q.cW = q.nW; q.cC = q.nC; q.cE = q.nE;
@@ -1212,9 +1232,6 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         const float nC = pin(x, y + 1), sC = pin(x, y - 1);
         const float nE = pin(x + 1, y + 1), nW = pin(x - 1, y + 1);
         const float sE = pin(x + 1, y - 1), sW = pin(x - 1, y - 1);
-        const float lap = (4.0f * (nC + sC + cE + cW)
-                           + 1.0f * (nE + nW + sE + sW)
-                           + -20.0f * cC) * (float)(1.0 / 6.0);
         const float gxv = 0.5f * (cE - cW);
         const float gyv = 0.5f * (nC - sC);
         const uint32_t qS = S_rd[(size_t)wrapi(gy0 + y, L) * P + wrapi(gx0 + x, L)];
@@ -1222,13 +1239,9 @@ __device__ void cell_global(const pfh2d_step_args& A, int n,
         float So = 0.0f;
         if (qS >= qc) So = (float)(qS - qc) * kQInv;
         else negc += 1u;
-        const float passive = gam * lap
-            - dwC * (cC * (1.0f - cC) * (1.0f - 2.0f * cC))
-            + volC * cC
-            - repC * cC * So;
-        const float adv = vx * gxv + vy * gyv;
-        const float rhs = (mrel == 1.0f) ? passive - adv : mrel * passive - adv;
-        const float pn = cC + dt * rhs;
+        const float passive = with_repulsion(
+            passive_head_of(nC, sC, cE, cW, nE, nW, sE, sW, cC, gam, dwC, volC), repC, cC, So);
+        const float pn = euler_update(cC, gxv, gyv, passive, vx, vy, mrel, dt);
         const int a = x - sx, b = y - sy;
         if ((unsigned)a < (unsigned)dwx && (unsigned)b < (unsigned)dwy)
             tout[(size_t)(dty0 + b) * kTilePitch + dtx0 + a] = pn;

@@ -225,7 +225,9 @@ def test_transpose_swaps_force_and_velocity(oracle, f64):
         wa = oracle.world_field(a.phi_now(), ca, L)
         wb = oracle.world_field(b.phi_now(), cb, L)
         assert np.abs(wb.transpose(0, 2, 1) - wa).max() <= tol_f
-        np.testing.assert_allclose(cb["Cx"], ca["Cy"], rtol=1e-9)
+        # binary32: the fused advection term vy gy + (vx gx) rounds the two
+        # axes differently, so the transposed run differs in last bits
+        np.testing.assert_allclose(cb["Cx"], ca["Cy"], rtol=1e-9 if f64 else 1e-8)
         assert np.array_equal(cb["gx0"], ca["gy0"]) and np.array_equal(cb["gy0"], ca["gx0"])
 
 
