@@ -23,8 +23,10 @@
 //               ascending order and commits the cell (centroid shift,
 //               support check) and its motion. The same launch zeroes the
 //               next aggregate buffer and snapshots the accepted cell state.
-//   k3_update   grid N*slabs, slab = a contiguous range of destination
-//               planes. Source planes stream through a two-buffer LDS ring
+//   k3_update   grid N*slabs*parts, slab = a contiguous range of
+//               destination planes, part = a contiguous range of their rows
+//               (two parts from B = 32). Source planes (the part's rows and
+//               one halo row on each side) stream through a two-buffer LDS ring
 //               with a zero margin of five, so the recentring shift (|s| <= 4)
 //               reads zeros outside the brick without branches. Each plane
 //               is reduced once to per-voxel plane sums (centre/edge/corner
@@ -566,22 +568,28 @@ __device__ __noinline__ void clip_voxel(const ClipArgs a, pfh3d_cell_state* cs,
     if (fabsf(nx) > kEps) raise3(cs, a.flags, PFH3D_FLAG_DESTINATION_CLIP);
 }
 
-__global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const int K) {
+__global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const int K,
+                                                  const int H) {
     extern __shared__ __align__(16) float lds[];
     __shared__ int s_abort;
 
     const int B = a.B, L = a.L, P = a.P;
     const int NT = (int)blockDim.x, tid = (int)threadIdx.x;
-    const int n = (int)blockIdx.x / K, k = (int)blockIdx.x - n * K;
+    // workgroup = (cell, z slab, y part): destination planes [zb, ze) and
+    // rows [yb, ye); parts share nothing but the input field and the atomics
+    const int KH = K * H;
+    const int n = (int)blockIdx.x / KH, rem = (int)blockIdx.x - n * KH;
+    const int k = rem / H, h = rem - (rem / H) * H;
     const int zb = k * B / K, ze = (k + 1) * B / K;
+    const int yb = h * B / H, ye = (h + 1) * B / H;
     pfh3d_cell_state* cs = a.cells + n;
     const size_t words = (size_t)B * B * B;
     const float* src = a.phi_in + (size_t)n * words;
     float* dst = a.phi_out + (size_t)n * words;
-    const int gpr = B / 4, G = B * B / 4;
+    const int gpr = B / 4, G = (ye - yb) * gpr;
     const bool act = tid < G;
-    const int gy_ = act ? tid / gpr : 0;
-    const int gx0 = act ? 4 * (tid - gy_ * gpr) : 0;
+    const int gy_ = act ? yb + tid / gpr : 0;
+    const int gx0 = act ? 4 * (tid % gpr) : 0;
     const int ox = wrap_origin(cs->origin_x, L);
     const int oy = wrap_origin(cs->origin_y, L);
     const int oz = wrap_origin(cs->origin_z, L);
@@ -592,7 +600,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                     abs(sz) > PFH3D_MAX_SHIFT)) {
             // a pending shift the measurement cannot produce (|s| <= 4):
             // counted once per cell, every slab keeps the accepted field
-            if (k == 0) raise3(cs, a.flags, PFH3D_FLAG_INVALID_GEOMETRY);
+            if (k == 0 && h == 0) raise3(cs, a.flags, PFH3D_FLAG_INVALID_GEOMETRY);
             ab = 1;
         }
         s_abort = ab;
@@ -615,7 +623,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
                                   v[j], cs, a.flags);
         }
         __syncthreads();
-        if (k == 0 && tid == 0) {
+        if (k == 0 && h == 0 && tid == 0) {
             cs->pending_shift_x = 0;
             cs->pending_shift_y = 0;
             cs->pending_shift_z = 0;
@@ -638,10 +646,14 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     const int PLU = (B + 2 * kMargin + 1) * PXU;      // B + 11 rows
     for (int i = tid; i < 2 * PLU; i += NT) lds[i] = 0.0f;
 
-    // staging: lane tid owns source group g = tid % gpr of row tid / gpr;
-    // element e of the group goes to sub-row e
+    // staging: lane tid owns source group tid % gpr of source row
+    // yb + sy - 1 + tid / gpr (the part's rows as the shift reads them, with
+    // one halo row on each side); element e of the group goes to sub-row e
     const int g = gx0 >> 2;
-    const int stage = (gy_ + kMargin) * PXU + kMarginG + g;
+    const int srow = yb + sy - 1 + tid / gpr;
+    const bool stg = tid < (ye - yb + 2) * gpr && (unsigned)srow < (unsigned)B;
+    const int sg = tid % gpr;
+    const int stage = (srow + kMargin) * PXU + kMarginG + sg;
     // reading: destination group (gx0.., gy_) reads source x = gx0 + sx + c - 1,
     // c = 0..5, of rows gy_ + sy - 1..gy_ + sy + 1: sub-row (sx + c - 1) & 3,
     // group g + ((sx + c - 1) >> 2), both offsets uniform
@@ -661,8 +673,9 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     };
     auto load_plane = [&](int p) -> f32x4 {
         f32x4 v = {0.f, 0.f, 0.f, 0.f};
-        if (act && (unsigned)p < (unsigned)B)
-            v = *reinterpret_cast<const PFH_GLOBAL f32x4*>(gsrc + (size_t)p * B * B + 4 * tid);
+        if (stg && (unsigned)p < (unsigned)B)
+            v = *reinterpret_cast<const PFH_GLOBAL f32x4*>(gsrc + (size_t)p * B * B +
+                                                          (size_t)srow * B + 4 * sg);
         return v;
     };
 
@@ -671,7 +684,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
     f32x4 R = load_plane(pbeg);
     f32x4 Rn = load_plane(pbeg + 1);
     __syncthreads();
-    if (act) stage_plane(lds, R);
+    if (stg) stage_plane(lds, R);
     __syncthreads();
 
     float T[4] = {0.f, 0.f, 0.f, 0.f}, Q1p[4] = {0.f, 0.f, 0.f, 0.f};
@@ -795,7 +808,7 @@ __global__ __launch_bounds__(1024) void k3_update(const pfh3d_step_args a, const
             gxm[j] = GX[j];
             gym[j] = GY[j];
         }
-        if (i + 1 < pcount && act) stage_plane(lds + (cur ^ 1) * PLU, Rn);
+        if (i + 1 < pcount && stg) stage_plane(lds + (cur ^ 1) * PLU, Rn);
         pfh::lds_barrier();   // the plane buffers are the only shared data here
         Rn = Rnn;
 #pragma unroll
@@ -1041,6 +1054,15 @@ int block_threads(int B) {
     const int g = B * B / 4;
     return ((g + 63) / 64) * 64;
 }
+// The update's y parts per slab: bricks from 32 split their rows in two, so
+// that a workgroup is at most 7 waves at B = 56 and two of them are resident
+// on a compute unit (one 13-wave workgroup otherwise): 208 against 212 us per
+// step at 256^3 / 128 cells.
+int update_parts(int B) { return B >= 32 ? 2 : 1; }
+int update_threads(int B) {
+    const int g = (B / update_parts(B) + 2) * (B / 4);   // rows + two halo rows
+    return ((g + 63) / 64) * 64;
+}
 size_t measure_lds(int B) { return 2u * (size_t)(B + 2) * (size_t)(B + 4) * sizeof(float); }
 size_t update_lds(int B) {
     return 2u * (size_t)(B + 2 * pfh3::kMargin + 1) * (size_t)pfh3::update_row_pitch(B) *
@@ -1083,11 +1105,11 @@ int pfh3d_update_slabs(int N, int B, int device) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
         "hipDeviceGetAttribute(multiProcessorCount)");
     if (st != PFH_OK) return st;
-    // one full round of two resident workgroups (of up to 1024 threads) per
-    // compute unit: a partial second round leaves half the chip idle (at
-    // 128 cells: 4 slabs 216 us per step, 6 slabs 228, 8 slabs 229);
-    // slabs of >= 4 planes
-    int k = (2 * (cus > 0 ? cus : 1)) / N;
+    // one full round of two resident workgroups per compute unit, counting
+    // the y parts of each slab: a partial round leaves part of the chip idle
+    // (at 128 cells, two parts: 2 slabs 208 us per step, 4 slabs 218, 8
+    // slabs 241); slabs of >= 4 planes
+    int k = (2 * (cus > 0 ? cus : 1)) / (N * update_parts(B));
     if (k < 1) k = 1;
     if (k > B / 4) k = B / 4;
     return k;
@@ -1144,8 +1166,9 @@ int pfh3d_update(const pfh3d_step_args* a, int slabs, void* stream) {
     }
     if (k > a->B) return invalid("slabs must lie in [1, B]");
     const size_t lds = update_lds(a->B);    // <= 46 KB: within the default 64 KB
-    hipLaunchKernelGGL(pfh3::k3_update, dim3((unsigned)(a->N * k)),
-                       dim3((unsigned)block_threads(a->B)), lds, as_stream(stream), *a, k);
+    const int H = update_parts(a->B);
+    hipLaunchKernelGGL(pfh3::k3_update, dim3((unsigned)(a->N * k * H)),
+                       dim3((unsigned)update_threads(a->B)), lds, as_stream(stream), *a, k, H);
     return launch_status("k3_update");
 }
 
