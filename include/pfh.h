@@ -717,8 +717,23 @@ int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out,
                           int N, void* stream);
 /* z slabs per cell that pfh3d_update uses when slabs == 0. */
 int pfh3d_update_slabs(int N, int B, int device);
+/* Replaces Sim3D::resize_cells (src/pf3d/adaptive_storage.cu:143-266) for a
+ * whole population: every cell's cube of edge B_old in phi_old is embedded,
+ * centred, in a cube of edge B_new in phi_new (offset (B_new - B_old) / 2 on
+ * each axis, zeros around it), and its state moved with it as
+ * translate_state does (origin - offset, Cx/Cy/Cz + offset V, support box +
+ * offset, pending shifts 0, the support flags cleared); the population's
+ * SUPPORT_EXHAUSTED and SUPPORT_EDGE counters are cleared. Values keep their
+ * world positions, so S is unchanged. 8 <= B_old < B_new <= 64, multiples
+ * of 8. */
+int pfh3d_grow_bricks(const float* phi_old, float* phi_new, pfh3d_cell_state* cells,
+                      uint32_t* flags, int N, int B_old, int B_new, void* stream);
 
 /* ---- 3D host-side set-up (no GPU work) ---- */
+/* next_brick_edge (include/pf3d/brick_classes.hpp:23-32) for domain side L:
+ * current + 32 rounded up to 16, at most the largest multiple of 8 below L
+ * and 64 (this build's largest brick); 0 when no larger brick fits. */
+int pfh3d_next_brick_edge(int current, int L);
 /* automatic_brick_edge (include/pf3d/params.cuh:122-152). */
 int pfh3d_brick_edge(double radius, double lambda);
 /* domain_side_for (include/pf3d/params.cuh:224-233). */

@@ -967,6 +967,58 @@ constexpr int kTraj3Words = (int)(sizeof(pfh3d_traj_cell) / 8);
 static_assert(sizeof(pfh3d_cell_state) % 16 == 0 && sizeof(pfh3d_traj_cell) % 8 == 0,
               "record vectors");
 
+// ---------------------------------------------------------------------------
+// brick growth: every cell's cube embedded, centred, in a larger cube
+// (Sim3D::resize_cells, src/pf3d/adaptive_storage.cu:143-266: copy_centered
+// :41-83, translate_state :85-106). new_local = old_local + off keeps every
+// value at its world position; the voxels outside the old cube are zero.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k3_grow_fields(const float* src, float* dst, int Bo,
+                                                      int Bn, int off) {
+    const int n = (int)blockIdx.x / Bn, z = (int)blockIdx.x - n * Bn;   // grid N * Bn
+    const int zs = z - off;
+    float* d = dst + ((size_t)n * Bn + z) * Bn * Bn;
+    const float* sp = src + (size_t)n * Bo * Bo * Bo;
+    for (int i = (int)threadIdx.x; i < Bn * Bn; i += (int)blockDim.x) {
+        const int y = i / Bn, x = i - y * Bn;
+        const int ys = y - off, xs = x - off;
+        float v = 0.0f;
+        if ((unsigned)xs < (unsigned)Bo && (unsigned)ys < (unsigned)Bo &&
+            (unsigned)zs < (unsigned)Bo)
+            v = sp[((size_t)zs * Bo + ys) * Bo + xs];
+        d[i] = v;
+    }
+}
+
+// The state of a moved cube: origin - off, first moments + off V on each
+// axis, a non-empty support box + off, no pending shift, the support flags
+// cleared (in the cell and, by thread 0, the population counters).
+__global__ __launch_bounds__(256) void k3_grow_state(pfh3d_cell_state* cells, uint32_t* flags,
+                                                     int N, int off) {
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n == 0) {
+        flags[PFH3D_FLAG_SUPPORT_EXHAUSTED] = 0u;
+        flags[PFH3D_FLAG_SUPPORT_EDGE] = 0u;
+    }
+    if (n >= N) return;
+    pfh3d_cell_state* c = cells + n;
+    c->origin_x -= off;
+    c->origin_y -= off;
+    c->origin_z -= off;
+    c->Cx += (double)off * c->V;
+    c->Cy += (double)off * c->V;
+    c->Cz += (double)off * c->V;
+    if (c->bb_hi_x >= c->bb_lo_x) {
+        c->bb_lo_x += off; c->bb_hi_x += off;
+        c->bb_lo_y += off; c->bb_hi_y += off;
+        c->bb_lo_z += off; c->bb_hi_z += off;
+    }
+    c->pending_shift_x = 0;
+    c->pending_shift_y = 0;
+    c->pending_shift_z = 0;
+    c->flags &= ~((1u << PFH3D_FLAG_SUPPORT_EXHAUSTED) | (1u << PFH3D_FLAG_SUPPORT_EDGE));
+}
+
 __global__ __launch_bounds__(64) void k3_pack_traj(const pfh3d_cell_state* cells,
                                                    pfh3d_traj_cell* out, int N) {
     __shared__ u32x4 in[64 * kCell3Vecs];
@@ -1233,7 +1285,36 @@ int pfh3d_pack_trajectory(const pfh3d_cell_state* cells, pfh3d_traj_cell* out, i
     return launch_status("k3_pack_traj");
 }
 
+int pfh3d_grow_bricks(const float* phi_old, float* phi_new, pfh3d_cell_state* cells,
+                      uint32_t* flags, int N, int B_old, int B_new, void* stream) {
+    if (!phi_old || !phi_new || phi_old == phi_new || !cells || !flags || N <= 0)
+        return invalid("grow_bricks buffers");
+    if (B_old < 8 || B_old % 8 != 0 || B_new % 8 != 0 || B_new <= B_old ||
+        B_new > PFH3D_MAX_EDGE)
+        return invalid("grow_bricks: edges multiples of 8 with 8 <= B_old < B_new <= 64");
+    const int off = (B_new - B_old) / 2;
+    hipLaunchKernelGGL(pfh3::k3_grow_fields, dim3((unsigned)(N * B_new)), dim3(256), 0,
+                       as_stream(stream), phi_old, phi_new, B_old, B_new, off);
+    int st = launch_status("k3_grow_fields");
+    if (st != PFH_OK) return st;
+    hipLaunchKernelGGL(pfh3::k3_grow_state, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), cells, flags, N, off);
+    return launch_status("k3_grow_state");
+}
+
 // ---- host-side set-up ----
+int pfh3d_next_brick_edge(int current, int L) {
+    // next_brick_edge (include/pf3d/brick_classes.hpp:23-32): 32 more planes,
+    // rounded up to 16, within the largest aligned edge below the side
+    // (src/pf3d/sim.cu:291-293) and this build's largest brick
+    if (current <= 0 || current % 8 != 0 || L <= 1) return 0;
+    int maximum = ((L - 1) / 8) * 8;
+    if (maximum > PFH3D_MAX_EDGE) maximum = PFH3D_MAX_EDGE;
+    if (maximum <= current) return 0;
+    const int rounded = ((current + 32 + 15) / 16) * 16;
+    return rounded < maximum ? rounded : maximum;
+}
+
 int pfh3d_brick_edge(double radius, double lambda) {
     if (!(radius > 0.0) || !(lambda > 0.0)) return invalid("radius/lambda");
     const double k = 2.7386127875258306 / lambda;

@@ -272,6 +272,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _P, _P]),
     "pfh3d_update_slabs": (_I, [_I, _I, _I]),
+    "pfh3d_grow_bricks": (_I, [_P, _P, _P, _P, _I, _I, _I, _P]),
+    "pfh3d_next_brick_edge": (_I, [_I, _I]),
     "pfh3d_brick_edge": (_I, [C.c_double, C.c_double]),
     "pfh3d_domain_side": (_I, [_I, C.c_double, C.c_double]),
     "pfh3d_calibrate_seed_radius": (_I, [C.c_double, C.c_double, _I, C.POINTER(C.c_double)]),

@@ -171,9 +171,9 @@ def workload_c3(brick_edge: int = 56) -> Workload3D:
     The automatic brick edge is 48, with no slack beyond the 8-voxel support
     margin: the relaxing diffuse tail raises SUPPORT_EXHAUSTED within tens of
     steps, which the reference answers by promoting cells to larger bricks
-    (adaptive support, outside this build). The throughput workload therefore
-    uses the reference's explicit --brick-edge 56, which holds the relaxed
-    support."""
+    (here Sim3D.run_adaptive grows the population to 64). The throughput
+    workload uses the reference's explicit --brick-edge 56, which holds the
+    relaxed support over the whole run."""
     lib = abi.load()
     side = int(lib.pfh3d_domain_side(128, 12.0, 0.05555))
     w = workload(f"pf3d_L256_N128_R12_B{brick_edge}", 128, 12.0, 3.5, side,

@@ -189,9 +189,15 @@ class Sim3D:
         refreshed (no tumble)."""
         from . import checkpoint3d
         info = checkpoint3d.probe(path)
-        if info["N"] != self.N or info["B"] != self.B or int(info["params"]["Nx"]) != self.L:
+        if info["N"] != self.N or info["B"] < self.B or int(info["params"]["Nx"]) != self.L:
             raise ValueError(f"{path}: checkpoint geometry (N={info['N']}, B={info['B']}) "
                              f"does not match this simulation (N={self.N}, B={self.B})")
+        if info["B"] != self.B:
+            # a population grown past this run's starting brick resumes at its edge
+            if info["B"] > abi.MAX_EDGE_3D or info["B"] % 8:
+                raise ValueError(f"{path}: brick edge {info['B']} is not supported")
+            self.sync()
+            self._set_brick(int(info["B"]))
         step = int(info["step"])
         slot = step % 2
         cells = np.zeros(self.N, abi.CELL3_DTYPE)
@@ -256,6 +262,72 @@ class Sim3D:
 
     def sync(self) -> None:
         torch.cuda.synchronize(self.dev)
+
+    # ---- adaptive bricks (src/pf3d/sim.cu:1653-1733) ----
+    def grow_bricks(self, B_new: int) -> None:
+        """Move the whole population into cubes of edge B_new (every value at
+        its world position, the states moved with their cubes, the support
+        flags cleared; S is unchanged). The graph, if any, is dropped: the
+        fields are new buffers."""
+        if not (self.B < B_new <= abi.MAX_EDGE_3D and B_new % 8 == 0):
+            raise ValueError(f"brick edge {B_new}: must exceed {self.B}, be a multiple of 8 "
+                             f"and at most {abi.MAX_EDGE_3D}")
+        self.sync()
+        slot = self.steps_done % 2
+        vox = B_new ** 3
+        new = [torch.zeros(self.N * vox, dtype=torch.float32, device=self.dev) for _ in range(2)]
+        abi.check(self.lib.pfh3d_grow_bricks(_ptr(self.phi[slot]), _ptr(new[slot]),
+                                             _ptr(self.cell), _ptr(self.flags), self.N, self.B,
+                                             B_new, self._sp()), "pfh3d_grow_bricks")
+        self.sync()
+        self._set_brick(B_new, new)
+
+    def _set_brick(self, B: int, phi=None) -> None:
+        """Fields of edge B (new zeroed buffers unless given) and the launch
+        shape and arguments that follow from it."""
+        self.B, self.voxels = int(B), int(B) ** 3
+        self.phi = phi if phi is not None else [
+            torch.zeros(self.N * self.voxels, dtype=torch.float32, device=self.dev)
+            for _ in range(2)]
+        dix = self.dev.index or 0
+        self.shards = min(self.shards, max(1, self.B // 4))
+        self.slabs = int(self.lib.pfh3d_update_slabs(self.N, self.B, dix))
+        self._args = [self._bake(s) for s in range(GRAPH_BODY)]
+        self._graph = None
+
+    def recover_support(self) -> bool:
+        """After steps rejected for SUPPORT_EXHAUSTED alone (the accepted state
+        kept), grow the population to the next brick tier; False when another
+        fatal flag is raised or no larger brick fits."""
+        f = self.flag_counts()
+        if f[abi.FLAG3_NAMES.index("support_exhausted")] == 0:
+            return False
+        if any(f[i] for i in range(abi.FLAG3_COUNT)
+               if (abi.FATAL3_MASK >> i) & 1 and abi.FLAG3_NAMES[i] != "support_exhausted"):
+            return False
+        nb = int(self.lib.pfh3d_next_brick_edge(self.B, self.L))
+        if nb <= self.B:
+            return False
+        self.grow_bricks(nb)
+        return True
+
+    def run_adaptive(self, steps: int, poll: int = 16) -> int:
+        """run() with the reference's support recovery, polled every `poll`
+        steps (src/pf3d/sim.cu:2142-2268): returns the number of growths;
+        stops early on a fatal flag it cannot recover from."""
+        end = self.steps_done + steps
+        grown = 0
+        graph = self._graph is not None
+        while self.steps_done < end:
+            self.run(min(poll, end - self.steps_done))
+            self.sync()
+            if self.fatal():
+                if not self.recover_support():
+                    break
+                grown += 1
+                if graph:
+                    self.capture_graph()
+        return grown
 
     # ---- readback ----
     def cells(self) -> np.ndarray:

@@ -116,3 +116,29 @@ def test_host_setup_draws_match_the_restated_placement():
     assert lib.pfh2d_domain_side(4, 12.0, 0.02) == 301
     assert lib.pfh2d_s_pitch(301) == 320 and lib.pfh2d_s_pitch(2048) == 2048
     assert lib.pfh2d_domain_side(0, 20.0, 0.5) < 0
+
+
+def _reference_next_brick_edge(current, maximum):
+    """next_brick_edge, include/pf3d/brick_classes.hpp:23-32 (alignment 8,
+    growth 32, tier quantum 16), restated."""
+    if current <= 0 or current % 8 or maximum <= current:
+        return 0
+    limit = maximum - maximum % 8
+    if limit <= current:
+        return 0
+    rounded = (current + 32 + 15) // 16 * 16
+    return min(rounded, limit)
+
+
+def test_next_brick_edge_follows_the_reference_tiers_within_64():
+    lib = abi.load()
+    for L in (49, 64, 65, 96, 97, 200, 256, 2048):
+        maximum = (L - 1) // 8 * 8            # src/pf3d/sim.cu:291-293
+        for cur in range(8, 72, 8):
+            want = _reference_next_brick_edge(cur, min(maximum, abi.MAX_EDGE_3D))
+            assert lib.pfh3d_next_brick_edge(cur, L) == want, (cur, L)
+    assert lib.pfh3d_next_brick_edge(48, 256) == 64      # the reference's 80, capped
+    assert lib.pfh3d_next_brick_edge(16, 256) == 48
+    assert lib.pfh3d_next_brick_edge(64, 256) == 0
+    assert lib.pfh3d_next_brick_edge(44, 256) == 0       # not an aligned edge
+    assert lib.pfh3d_grow_bricks(None, None, None, None, 1, 40, 56, None) < 0
