@@ -327,50 +327,98 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
     const int PX = B + 4, PL = (B + 2) * PX;
     const int own = (y + 1) * PX + 4 + x0;
 
-    for (int i = tid; i < 2 * PL; i += NT) lds[i] = 0.0f;
+    // A ring of four planes in LDS, each with a zero border (x/y neighbours
+    // past the brick): the plane a lane measures, its two z neighbours, and
+    // the plane being filled. Lanes only ever write their own 16 B of a
+    // plane's interior, so the border stays zero.
+    for (int i = tid; i < 4 * PL; i += NT) lds[i] = 0.0f;
+    auto slot = [&](int p) -> float* { return lds + ((p - zb + 1) & 3) * PL; };
 
+    // Every vector load of the plane loop is unconditional and the loop
+    // stores nothing to global memory, so the compiler's wait before each
+    // use counts only the loads issued after it: a plane arrives one
+    // iteration after it is requested, the aggregate words one plane ahead
+    // of their use, and no iteration waits for its own requests. Planes
+    // outside the brick are read from a clamped address and replaced by
+    // zeros; inactive lanes read group 0.
     const PFH_GLOBAL float* tile = gptr(a.phi_in) + (size_t)n * B * B * B;
     const PFH_GLOBAL uint32_t* Sin = gptr(a.S_in);
-    auto load_plane = [&](int p) -> f32x4 {
-        f32x4 v = {0.f, 0.f, 0.f, 0.f};
-        if (act && (unsigned)p < (unsigned)B)
-            v = *reinterpret_cast<const PFH_GLOBAL f32x4*>(tile + (size_t)p * B * B + 4 * tid);
-        return v;
+    const int lane_off = act ? 4 * tid : 0;
+    // (the raw request, and the zeros applied where the plane is used, so
+    // that nothing touches the loaded registers before then)
+    auto request_plane = [&](int p) -> f32x4 {
+        const int pc = p < 0 ? 0 : (p >= B ? B - 1 : p);
+        return *reinterpret_cast<const PFH_GLOBAL f32x4*>(tile + (size_t)pc * B * B +
+                                                          lane_off);
     };
-    f32x4 Am = load_plane(zb - 1);
-    f32x4 A0 = load_plane(zb);
-    f32x4 Ap = load_plane(zb + 1);
-    __syncthreads();
-    if (act) *reinterpret_cast<f32x4*>(lds + own) = A0;
+    auto in_brick = [&](const f32x4& v, int p) -> f32x4 {
+        const bool in = act && (unsigned)p < (unsigned)B;
+        return f32x4{in ? v[0] : 0.f, in ? v[1] : 0.f, in ? v[2] : 0.f, in ? v[3] : 0.f};
+    };
+    auto load_plane = [&](int p) -> f32x4 { return in_brick(request_plane(p), p); };
+    const int wy = wrap_off(oy, y, L);
+    int wx[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wx[j] = wrap_off(ox, x0 + j, L);
+    // S words of this lane's 4 voxels in plane z, read only where that
+    // plane's phi is nonzero (the interaction reads S nowhere else, and most
+    // of a brick is empty); the other requests go to word 0, one cache line
+    // for the whole launch. The plane loop reads a word only where its
+    // voxel's phi is nonzero, the same mask.
+    auto load_s = [&](int z, const f32x4& ph, uint32_t (&sv)[4]) {
+        const size_t srow = s_index(L, P, 0, wy, wrap_off(oz, z < B ? z : B - 1, L));
+        const bool zin = act && z < ze;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t idx = (zin && ph[j] != 0.0f) ? srow + (size_t)wx[j] : 0u;
+            sv[j] = Sin[idx];
+        }
+    };
+
+    {
+        const f32x4 Pm = load_plane(zb - 1);
+        const f32x4 P0 = load_plane(zb);
+        const f32x4 P1 = load_plane(zb + 1);
+        __syncthreads();                        // the ring's zeros are written
+        if (act) {
+            *reinterpret_cast<f32x4*>(slot(zb - 1) + own) = Pm;
+            *reinterpret_cast<f32x4*>(slot(zb) + own) = P0;
+            *reinterpret_cast<f32x4*>(slot(zb + 1) + own) = P1;
+        }
+    }
+    // Two planes in flight (z + 2 and z + 3 while plane z is measured) and
+    // the aggregate words of two planes (z and z + 1): each request has two
+    // iterations to arrive.
+    // (issued in the order the loop issues them: S0, R0, S1, R1)
+    uint32_t S0[4], S1[4];
+    f32x4 R0, R1;
+    int Rp0 = zb + 2, Rp1 = zb + 3;            // the planes' indices
+    {
+        const f32x4 P0 = *reinterpret_cast<const f32x4*>(slot(zb) + own);
+        const f32x4 P1 = *reinterpret_cast<const f32x4*>(slot(zb + 1) + own);
+        load_s(zb, P0, S0);
+        R0 = request_plane(Rp0);
+        load_s(zb + 1, P1, S1);
+        R1 = request_plane(Rp1);
+    }
     __syncthreads();
 
     double aV = 0.0, aCx = 0.0, aCy = 0.0, aCz = 0.0, aS = 0.0, aIx = 0.0, aIy = 0.0,
            aIz = 0.0;
     int lo0 = B, lo1 = B, lo2 = B, hi0 = -1, hi1 = -1, hi2 = -1;
-    uint32_t pmax = 0u;
-    const int wy = wrap_off(oy, y, L);
-    // S words of this lane's 4 voxels in plane z, fetched one plane ahead and
-    // only where that plane's phi (already in registers) is nonzero: the
-    // interaction reads S nowhere else, and most of a brick is empty
-    auto load_s = [&](int z, const f32x4& ph, uint32_t (&sv)[4]) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sv[j] = 0u;
-        if (act && z < ze) {
-            const size_t srow = s_index(L, P, 0, wy, wrap_off(oz, z, L));
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (ph[j] != 0.0f) sv[j] = Sin[srow + (size_t)wrap_off(ox, x0 + j, L)];
-        }
-    };
-    uint32_t Sc[4], Sn[4];
-    load_s(zb, A0, Sc);
+    uint32_t pmax = 0u, n_nonfin = 0u, n_neg = 0u;
 
-    for (int z = zb; z < ze; ++z) {
-        const int cur = (z - zb) & 1;
-        const f32x4 An = (z + 2 <= ze) ? load_plane(z + 2) : f32x4{0.f, 0.f, 0.f, 0.f};
-        load_s(z + 1, Ap, Sn);
+    // One plane: measure plane z with the aggregate words requested two
+    // planes earlier (set Sz), store the plane that arrived (z + 2, in Rz)
+    // into the ring, request its aggregate words into Sz and plane z + 4
+    // into Rz.
+    auto plane_step = [&](int z, uint32_t (&Sz)[4], f32x4& Rz, int& Rpz) {
+        const uint32_t (&Sc)[4] = Sz;
         if (act) {
-            const float* pl = lds + cur * PL;
+            const float* pl = slot(z);
+            const f32x4 A0 = *reinterpret_cast<const f32x4*>(pl + own);
+            const f32x4 Ap = *reinterpret_cast<const f32x4*>(slot(z + 1) + own);
+            const f32x4 Am = *reinterpret_cast<const f32x4*>(slot(z - 1) + own);
             const f32x4 ym = *reinterpret_cast<const f32x4*>(pl + own - PX);
             const f32x4 yp = *reinterpret_cast<const f32x4*>(pl + own + PX);
             const float xl = pl[own - 1], xr = pl[own + 4];
@@ -378,7 +426,7 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
             for (int j = 0; j < 4; ++j) {
                 const float c = A0[j];
                 if (!isfinite(c)) {
-                    raise3(cs, a.flags, PFH3D_FLAG_NONFINITE);
+                    ++n_nonfin;
                     continue;
                 }
                 const float xm = j == 0 ? xl : A0[j == 0 ? 0 : j - 1];
@@ -389,7 +437,13 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
                 if (c == 0.0f && gx == 0.0f && gy == 0.0f && gz == 0.0f) continue;
                 float other = 0.0f;
                 const int x = x0 + j;
-                if (c != 0.0f) other = other_field(Sc[j], c, cs, a.flags);
+                if (c != 0.0f) {
+                    // other_field (src/pf3d/device_common.cuh:182-191), its
+                    // S_NEGATIVE event counted here and raised after the loop
+                    const uint32_t agg = Sc[j], mine = q_of(c);
+                    if (agg >= mine) other = (float)(agg - mine) * pfh::kQInv;
+                    else ++n_neg;
+                }
                 const double sq = (double)c * (double)c;
                 aV += sq;
                 aCx += sq * (double)x;
@@ -410,13 +464,31 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
                 }
             }
         }
-        if (z + 1 < ze && act) *reinterpret_cast<f32x4*>(lds + (cur ^ 1) * PL + own) = Ap;
-        pfh::lds_barrier();   // the plane buffers are the only shared data here
-        Am = A0;
-        A0 = Ap;
-        Ap = An;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Sc[j] = Sn[j];
+        // plane z + 2 replaces plane z - 2, last read before the previous
+        // barrier; a plane past the shard's last neighbour (ze) is never
+        // read, and its request is the cached plane ze again
+        const f32x4 An = in_brick(Rz, Rpz);
+        if (act) *reinterpret_cast<f32x4*>(slot(z + 2) + own) = An;
+        load_s(z + 2, An, Sz);
+        Rpz = z + 4 <= ze ? z + 4 : ze;
+        Rz = request_plane(Rpz);
+        pfh::lds_barrier();   // the ring is the only data shared here
+    };
+    // pairs of planes, then an odd last plane: both steps of a pair run
+    // unconditionally, so the request counts the waits see are fixed
+    int z = zb;
+    for (; z + 1 < ze; z += 2) {
+        plane_step(z, S0, R0, Rp0);
+        plane_step(z + 1, S1, R1, Rp1);
+    }
+    if (z < ze) plane_step(z, S0, R0, Rp0);
+    if (n_nonfin) {
+        atomicOr(&cs->flags, 1u << PFH3D_FLAG_NONFINITE);
+        atomicAdd(&a.flags[PFH3D_FLAG_NONFINITE], n_nonfin);
+    }
+    if (n_neg) {
+        atomicOr(&cs->flags, 1u << PFH3D_FLAG_S_NEGATIVE);
+        atomicAdd(&a.flags[PFH3D_FLAG_S_NEGATIVE], n_neg);
     }
 
     // 64-lane trees, then waves in ascending order
@@ -1115,7 +1187,7 @@ int update_threads(int B) {
     const int g = (B / update_parts(B) + 2) * (B / 4);   // rows + two halo rows
     return ((g + 63) / 64) * 64;
 }
-size_t measure_lds(int B) { return 2u * (size_t)(B + 2) * (size_t)(B + 4) * sizeof(float); }
+size_t measure_lds(int B) { return 4u * (size_t)(B + 2) * (size_t)(B + 4) * sizeof(float); }
 size_t update_lds(int B) {
     return 2u * (size_t)(B + 2 * pfh3::kMargin + 1) * (size_t)pfh3::update_row_pitch(B) *
            sizeof(float);
@@ -1140,7 +1212,16 @@ int launch_measure(const pfh3d_step_args* a, const pfh3::MeasureMode& m, void* s
     if (a->max_shift < 0 || a->max_shift > PFH3D_MAX_SHIFT) return invalid("max_shift");
     if (m.clear_out && !a->S_out) return invalid("S_out");
     if (m.snapshot && !a->accepted) return invalid("accepted");
-    const size_t lds = measure_lds(a->B);   // <= 36 KB: within the default 64 KB
+    // four ring planes: 55.7 KB at B = 56, 71.8 KB at B = 64 (above the
+    // default 64 KB of dynamic LDS, so the limit is raised for the kernel)
+    const size_t lds = measure_lds(a->B);
+    if (lds > 65536u) {
+        st = hip_status(hipFuncSetAttribute(reinterpret_cast<const void*>(&pfh3::k3_measure),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)lds),
+                        "hipFuncSetAttribute(k3_measure)");
+        if (st != PFH_OK) return st;
+    }
     hipLaunchKernelGGL(pfh3::k3_measure, dim3((unsigned)(a->N * a->shards)),
                        dim3((unsigned)block_threads(a->B)), lds, as_stream(stream), *a, m);
     return launch_status("k3_measure");
