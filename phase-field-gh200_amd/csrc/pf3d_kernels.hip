@@ -412,13 +412,15 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
     // planes earlier (set Sz), store the plane that arrived (z + 2, in Rz)
     // into the ring, request its aggregate words into Sz and plane z + 4
     // into Rz.
+    // This lane's own groups of planes z - 1 and z roll through registers
+    // (read from the ring once each, as plane z + 1)
+    f32x4 Am = *reinterpret_cast<const f32x4*>(slot(zb - 1) + own);
+    f32x4 A0 = *reinterpret_cast<const f32x4*>(slot(zb) + own);
     auto plane_step = [&](int z, uint32_t (&Sz)[4], f32x4& Rz, int& Rpz) {
         const uint32_t (&Sc)[4] = Sz;
+        const f32x4 Ap = *reinterpret_cast<const f32x4*>(slot(z + 1) + own);
         if (act) {
             const float* pl = slot(z);
-            const f32x4 A0 = *reinterpret_cast<const f32x4*>(pl + own);
-            const f32x4 Ap = *reinterpret_cast<const f32x4*>(slot(z + 1) + own);
-            const f32x4 Am = *reinterpret_cast<const f32x4*>(slot(z - 1) + own);
             const f32x4 ym = *reinterpret_cast<const f32x4*>(pl + own - PX);
             const f32x4 yp = *reinterpret_cast<const f32x4*>(pl + own + PX);
             const float xl = pl[own - 1], xr = pl[own + 4];
@@ -467,6 +469,8 @@ __global__ __launch_bounds__(1024) void k3_measure(const pfh3d_step_args a,
         // plane z + 2 replaces plane z - 2, last read before the previous
         // barrier; a plane past the shard's last neighbour (ze) is never
         // read, and its request is the cached plane ze again
+        Am = A0;
+        A0 = Ap;
         const f32x4 An = in_brick(Rz, Rpz);
         if (act) *reinterpret_cast<f32x4*>(slot(z + 2) + own) = An;
         load_s(z + 2, An, Sz);
