@@ -1062,28 +1062,63 @@ __device__ void decide_cell(
     const float* tn = phi_now + (size_t)n * pfh::kTileArea;
     const float* pr = pred + (size_t)n * kCropPix;
     double e = 0.0, t = 0.0, inc = 0.0;
-    for (int p = (int)threadIdx.x; p < kCropPix; p += NT) {
-        const int j = p / kCrop, i = p - j * kCrop;
-        int wx = ox + i, wy = oy + j;
-        if (wx >= L) wx -= L;
-        if (wy >= L) wy -= L;
-        const double c = (double)window_value(tc, sc, gxc, gyc, wx, wy, L);
-        const double v = (double)window_value(tn, sn, gxn, gyn, wx, wy, L);
-        const double d = (double)pr[p] - c;
-        e += d * d;
-        t += c * c;
-        inc += (c - v) * (c - v);
+    // Each thread visits its pixels in ascending order, U at a time: the U
+    // pixels' loads are issued together, then added in order.
+    constexpr int U = 4;
+    for (int p0 = (int)threadIdx.x; p0 < kCropPix; p0 += U * NT) {
+        float cf[U], vf[U], pf[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT;
+            cf[u] = vf[u] = pf[u] = 0.0f;
+            if (p < kCropPix) {
+                const int j = p / kCrop, i = p - j * kCrop;
+                int wx = ox + i, wy = oy + j;
+                if (wx >= L) wx -= L;
+                if (wy >= L) wy -= L;
+                cf[u] = window_value(tc, sc, gxc, gyc, wx, wy, L);
+                vf[u] = window_value(tn, sn, gxn, gyn, wx, wy, L);
+                pf[u] = pr[p];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (p0 + u * NT < kCropPix) {
+                const double c = (double)cf[u], v = (double)vf[u];
+                const double d = (double)pf[u] - c;
+                e += d * d;
+                t += c * c;
+                inc += (c - v) * (c - v);
+            }
+        }
     }
     // gxc, ox in [0, L) and a < wx < L, so i lies in (-L, 2L): one wrap
-    for (int p = (int)threadIdx.x; p < sc.wx * sc.wy; p += NT) {
-        const int b = p / sc.wx, a = p - b * sc.wx;
-        int i = gxc + a - ox, j = gyc + b - oy;
-        i += i < 0 ? L : (i >= L ? -L : 0);
-        j += j < 0 ? L : (j >= L ? -L : 0);
-        if (i < kCrop && j < kCrop) continue;          // counted on the crop
-        const double c = (double)tc[(size_t)(sc.ty0 + b) * pfh::kTilePitch + sc.tx0 + a];
-        e += c * c;
-        t += c * c;
+    const int wpix = sc.wx * sc.wy;
+    for (int p0 = (int)threadIdx.x; p0 < wpix; p0 += U * NT) {
+        float cf[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT;
+            on[u] = false;
+            cf[u] = 0.0f;
+            if (p < wpix) {
+                const int b = p / sc.wx, a = p - b * sc.wx;
+                int i = gxc + a - ox, j = gyc + b - oy;
+                i += i < 0 ? L : (i >= L ? -L : 0);
+                j += j < 0 ? L : (j >= L ? -L : 0);
+                on[u] = !(i < kCrop && j < kCrop);    // crop pixels are counted above
+                if (on[u]) cf[u] = tc[(size_t)(sc.ty0 + b) * pfh::kTilePitch + sc.tx0 + a];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (on[u]) {
+                const double c = (double)cf[u];
+                e += c * c;
+                t += c * c;
+            }
+        }
     }
     e = pfh::wave_sum(e);
     t = pfh::wave_sum(t);
@@ -1151,17 +1186,30 @@ __device__ void commit_staged(float* t, const float* pr, int L, int P, uint32_t*
     constexpr int CH = 8;
     for (int k0 = 0; k0 < RB; k0 += CH) {
         uint32_t olds[CH];
+        // the chunk's values are read before any of its stores (the tile
+        // written may be the buffer read, as far as the compiler knows), so
+        // that the reads are in flight together: the prediction on the crop,
+        // or (src) a whole window's field
+        float vals[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            vals[k] = 0.0f;
+            const int y = y0 + k0 + k;
+            if (k0 + k < RB && y < WY) {
+                const int wy = pfh::wrap_once(gy0 + y, L);
+                int j = wy - oy;
+                if (j < 0) j += L;
+                vals[k] = src ? src[(size_t)y * pfh::kTilePitch + x]
+                              : ((icrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
             olds[k] = 0u;
             const int y = y0 + k0 + k;
             if (k0 + k < RB && y < WY) {
                 const int wy = pfh::wrap_once(gy0 + y, L);
-                int j = wy - oy;
-                if (j < 0) j += L;
-                // the prediction on the crop, or (src) a whole window's field
-                const float v = src ? src[(size_t)y * pfh::kTilePitch + x]
-                                    : ((icrop && j < kCrop) ? pr[j * kCrop + i] : 0.0f);
+                const float v = vals[k];
                 t[(size_t)y * pfh::kTilePitch + x] = v;
                 const uint32_t q = pfh::q_of(v);
                 if (q) {
