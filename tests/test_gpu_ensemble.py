@@ -133,3 +133,36 @@ def test_autotuned_ensemble_and_3d_shapes_keep_results():
     sim.sync()
     assert np.array_equal(sim.phi_now().view(np.uint32), o3.phi_now().view(np.uint32))
     assert sim.cells().tobytes() == o3.cells.tobytes()
+
+
+def test_c4_long_run_S_exact():
+    """The 64-replica ensemble (sides 256/512/1024, seeds 100..163) over its
+    full 5000 steps (5004: whole six-step graphs): no fatal flag, every
+    replica's field finite and bounded, and every replica's aggregate S equal
+    bit for bit to the oracle's scatter of that replica's stored fields (no
+    drift from the incremental scatter into the shared slot arenas or from
+    the arena-wide clear-ahead)."""
+    from oracle import pfo
+    from pfh import ensemble
+    lib = pfo.load()
+    wls = params.ensemble_c4()
+    ens = ensemble.Ensemble2D(wls)
+    ens.capture_graph()
+    say(f"C4: {ens.N} cells in {ens.R} replicas, 5004 steps from six-step graphs")
+    ens.run(5004)
+    ens.sync()
+    assert ens.steps_done == 5004
+    assert not ens.fatal(), ens.flag_counts()
+    for r in range(ens.R):
+        phi = np.ascontiguousarray(ens.replica_phi(r))
+        assert np.isfinite(phi).all(), f"replica {r}"
+        assert phi.max() <= 1.5 and phi.min() >= -0.5 and phi.max() > 0.9, f"replica {r}"
+        cells = ens.replica_cells(r)     # bound: the oracle reads through their addresses
+        cls = ens.replica_classes(r)
+        L, P = ens.Ls[r], ens.Ps[r]
+        S = np.zeros(L * P, np.uint32)
+        flags = np.zeros(8, np.uint32)
+        lib.pfo2d_scatter_all(phi.ctypes.data, cells.ctypes.data, cls.ctypes.data,
+                              S.ctypes.data, ens.Ns[r], L, P, flags.ctypes.data)
+        assert np.array_equal(S, ens.replica_S(r)), f"S of replica {r} (side {L})"
+    say("C4: every replica's S exact")
