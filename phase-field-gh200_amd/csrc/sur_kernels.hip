@@ -800,70 +800,170 @@ __global__ __launch_bounds__(256) void k_sur_out_bwd(const float* g, const bf16_
         const size_t o = ((size_t)n * kCropPix + q) * kCH + 8 * c8;
         for (int k = 0; k < 8; ++k) dH[o + k] = f2bf(bf2f(H[o + k]) > 0.0f ? acc[k] : 0.0f);
     }
-    // dW4[t][ci] = sum_p g[p] h3[p + off(t)][ci] (entries 0..287), db4 (entry 288)
-    for (int e = tid; e < 9 * kCH + 1; e += 256) {
-        const int tt = e / kCH, cc = e - tt * kCH;
-        float a = 0.0f;
-        for (int y = 0; y < kCrop; ++y)
-            for (int x = 0; x < kCrop; ++x) {
-                const float gv = gn[y * kCrop + x];
-                if (tt == 9) {
-                    a += gv;
-                    continue;
-                }
-                const int yy = y + tt / 3 - 1, xx = x + tt % 3 - 1;
-                if ((unsigned)yy >= (unsigned)kCrop || (unsigned)xx >= (unsigned)kCrop) continue;
-                a += gv * bf2f(H[(((size_t)n * kCrop + yy) * kCrop + xx) * kCH + cc]);
+    // dW4[t][ci] = sum_p g[p] h3[p + off(t)][ci] = sum_q g[q - off(t)] h3[q][ci]
+    // (entries 0..287) and db4 = sum_p g[p] (entry 288), a row q_y at a time:
+    // the three rows of g it pairs with (zero outside the crop) and its row
+    // of h3 staged in LDS, each thread summing its entries over the row in
+    // pixel order
+    __shared__ float gl[3 * kHaloCols];
+    __shared__ float hl[kCrop * kCH];
+    float a0 = 0.0f, a1 = 0.0f;                    // entries tid and tid + 256
+    for (int y = 0; y < kCrop; ++y) {
+        __syncthreads();
+        for (int i = tid; i < 3 * kHaloCols; i += 256) {
+            const int r = i / kHaloCols, xx = i - r * kHaloCols - 1;
+            const int yy = y + 1 - r;                   // row r pairs with ky = r
+            gl[i] = ((unsigned)yy < (unsigned)kCrop && (unsigned)xx < (unsigned)kCrop)
+                        ? gn[yy * kCrop + xx] : 0.0f;
+        }
+        for (int i = tid; i < kCrop * kCH; i += 256)
+            hl[i] = bf2f(H[((size_t)n * kCrop + y) * kCrop * kCH + i]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = tid + 256 * k;
+            if (e > 9 * kCH) break;
+            float a = k == 0 ? a0 : a1;
+            if (e < 9 * kCH) {
+                const int t = e / kCH, cc = e - t * kCH;
+                const int ky = t / 3, kx = t - 3 * ky;
+                // g[y - ky + 1][x - kx + 1] sits at gl[ky][x - kx + 1 + 1]
+                const float* gr = gl + ky * kHaloCols + 2 - kx;
+                for (int x = 0; x < kCrop; ++x) a += gr[x] * hl[x * kCH + cc];
+            } else {
+                const float* gr = gl + kHaloCols + 1;   // g row y
+                for (int x = 0; x < kCrop; ++x) a += gr[x];
             }
-        dW4_part[(size_t)n * (9 * kCH + 1) + e] = a;
+            if (k == 0) a0 = a; else a1 = a;
+        }
     }
+    dW4_part[(size_t)n * (9 * kCH + 1) + tid] = a0;
+    if (tid + 256 <= 9 * kCH) dW4_part[(size_t)n * (9 * kCH + 1) + tid + 256] = a1;
 }
 
 // Per-crop partials of dW[co][tap][ci] and db[co] for an MFMA layer:
 // dW = sum_p dY[p][co] X[p + off(tap)][ci], db = sum_p dY[p][co].
+// dW is a GEMM on MFMA (v_mfma_f32_32x32x16_bf16, FP32 accumulation) with
+// M = 32 output channels, N = 9 CIN (tap, input channel) in 32-column tiles
+// and K = the crop's pixels, one crop row (six 16-pixel steps) at a time.
+// Every operand is eight consecutive pixels of one channel, one 16 B LDS
+// read: dY's row is staged channel-major, and each row of X is staged once
+// as three x-shifted, channel-major copies (one per kx) in a ring of three
+// rows, so that the taps of output row y read X rows y - 1, y, y + 1 from
+// the ring without rebuilding them. db is summed on the VALU in pixel
+// order, one lane per channel.
 template <int CIN>
 __global__ __launch_bounds__(256) void k_sur_wgrad(const bf16_t* dY, const bf16_t* X,
                                                    float* part) {
-    constexpr int NW = kCH * 9 * CIN;
-    const int n = (int)blockIdx.x, tid = (int)threadIdx.x;
-    __shared__ float dyl[kCrop * kCH];            // one row of dY (fp32)
-    __shared__ float xl[3 * kHaloCols * CIN];     // three rows of X with a zero halo
-    constexpr int PER = (NW + kCH + 255) / 256;
-    float acc[PER];
-    for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
-    for (int y = 0; y < kCrop; ++y) {
-        __syncthreads();
-        for (int i = tid; i < kCrop * kCH; i += 256)
-            dyl[i] = bf2f(dY[((size_t)n * kCrop + y) * kCrop * kCH + i]);
-        for (int i = tid; i < 3 * kHaloCols * CIN; i += 256) {
-            const int r = i / (kHaloCols * CIN), rem = i - r * kHaloCols * CIN;
-            const int cc = rem / CIN, ci = rem - cc * CIN;
-            const int yy = y + r - 1, xx = cc - 1;
-            xl[i] = ((unsigned)yy < (unsigned)kCrop && (unsigned)xx < (unsigned)kCrop)
-                        ? bf2f(X[(((size_t)n * kCrop + yy) * kCrop + xx) * CIN + ci]) : 0.0f;
-        }
-        __syncthreads();
-        for (int k = 0; k < PER; ++k) {
-            const int e = tid + 256 * k;
-            if (e >= NW + kCH) break;
-            float a = acc[k];
-            if (e < NW) {
-                const int co = e / (9 * CIN), rem = e - co * 9 * CIN;
-                const int t = rem / CIN, ci = rem - t * CIN;
-                const int ky = t / 3, kx = t - 3 * ky;
-                const float* xr = xl + (ky * kHaloCols + kx) * CIN + ci;
-                for (int x = 0; x < kCrop; ++x) a += dyl[x * kCH + co] * xr[x * CIN];
-            } else {
-                const int co = e - NW;
-                for (int x = 0; x < kCrop; ++x) a += dyl[x * kCH + co];
+    constexpr int N = 9 * CIN;                  // columns: tap x input channel
+    constexpr int NTL = (N + 31) / 32;          // 32-column tiles: 9 or 3
+    constexpr int RP = kCrop + 8;               // LDS row pitch (bf16): 208 B
+    constexpr int BLK = 3 * CIN * RP;           // one X row: [kx][ci][pixel]
+    __shared__ __align__(16) bf16_t xb[3 * BLK];
+    __shared__ __align__(16) bf16_t dyT[kCH * RP];
+    __shared__ __align__(16) bf16_t zrow[RP];
+    const int crop = (int)blockIdx.x, tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    // The ring starts zero: entries (kx = 0, pixel 0) and (kx = 2, pixel 95)
+    // are the crop's zero padding and are never written again.
+    for (int i = tid; i < 3 * BLK; i += 256) xb[i] = 0;
+    for (int i = tid; i < RP; i += 256) zrow[i] = 0;
+    const bf16_t* Xc = X + (size_t)crop * kCropPix * CIN;
+    const bf16_t* dYc = dY + (size_t)crop * kCropPix * kCH;
+    __syncthreads();
+
+    // X row r (zeros outside the crop) into ring slot (r + 3) % 3: pixel xx
+    // of channel ci lands at column xx - kx + 1 of copy kx
+    auto build = [&](int r) {
+        bf16_t* b = xb + ((r + 3) % 3) * BLK;
+        constexpr int C8 = CIN / 8;
+        const bool in = (unsigned)r < (unsigned)kCrop;
+        for (int i = tid; i < kCrop * C8; i += 256) {
+            const int xx = i / C8, c8 = i - (i / C8) * C8;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (in)
+                v = *reinterpret_cast<const u32x4*>(Xc + ((size_t)r * kCrop + xx) * CIN + 8 * c8);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const bf16_t h = (bf16_t)(v[k >> 1] >> (16 * (k & 1)));
+                const int ci = 8 * c8 + k;
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) {
+                    const int px = xx - kx + 1;
+                    if ((unsigned)px < (unsigned)kCrop) b[(kx * CIN + ci) * RP + px] = h;
+                }
             }
-            acc[k] = a;
+        }
+    };
+    // dY row y, channel-major
+    auto stage_dy = [&](int y) {
+        for (int i = tid; i < kCrop * (kCH / 8); i += 256) {
+            const int px = i >> 2, c8 = i & 3;
+            const u32x4 v = *reinterpret_cast<const u32x4*>(
+                dYc + ((size_t)y * kCrop + px) * kCH + 8 * c8);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                dyT[(8 * c8 + k) * RP + px] = (bf16_t)(v[k >> 1] >> (16 * (k & 1)));
+        }
+    };
+
+    f32x16 acc0 = {}, acc1 = {}, acc2 = {};    // tiles wave, wave + 4, wave + 8
+    float db = 0.0f;
+    build(-1);
+    build(0);
+    for (int y = 0; y < kCrop; ++y) {
+        __syncthreads();            // row y - 1's operand reads are done
+        build(y + 1);               // into the slot of row y - 2
+        stage_dy(y);
+        __syncthreads();
+        if (tid < kCH) {
+            for (int px = 0; px < kCrop; ++px) db += bf2f(dyT[tid * RP + px]);
+        }
+        const bf16_t* brow[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int nn = 32 * (wave + 4 * j) + col;
+            brow[j] = zrow;
+            if (nn < N) {
+                const int t = nn / CIN, ci = nn - (nn / CIN) * CIN;
+                const int ky = t / 3, kx = t - 3 * (t / 3);
+                brow[j] = xb + ((y + ky + 2) % 3) * BLK + (kx * CIN + ci) * RP;
+            }
+        }
+        const bf16_t* arow = dyT + col * RP;
+#pragma unroll
+        for (int ks = 0; ks < kCrop / 16; ++ks) {
+            const int k0 = 16 * ks + 8 * half;
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + k0);
+            if (wave < NTL)
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    av, *reinterpret_cast<const bf16x8*>(brow[0] + k0), acc0, 0, 0, 0);
+            if (wave + 4 < NTL)
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    av, *reinterpret_cast<const bf16x8*>(brow[1] + k0), acc1, 0, 0, 0);
+            if (wave + 8 < NTL)
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    av, *reinterpret_cast<const bf16x8*>(brow[2] + k0), acc2, 0, 0, 0);
         }
     }
-    for (int k = 0; k < PER; ++k) {
-        const int e = tid + 256 * k;
-        if (e < NW + kCH) part[(size_t)n * (NW + kCH) + e] = acc[k];
+
+    // lane holds column 32 T + col of tile T, output channels 8 (r >> 2) +
+    // 4 half + (r & 3)
+    float* out = part + (size_t)crop * (kCH * N + kCH);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int T = wave + 4 * j;
+        const int nn = 32 * T + col;
+        if (T >= NTL || nn >= N) continue;
+        const f32x16 a = j == 0 ? acc0 : (j == 1 ? acc1 : acc2);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int co = 8 * (r >> 2) + 4 * half + (r & 3);
+            out[(size_t)co * N + nn] = a[r];
+        }
     }
+    if (tid < kCH) out[kCH * N + tid] = db;
 }
 
 // Fixed-order sum of per-crop partials: out[e] = sum_n part[n][e] (ascending n)
